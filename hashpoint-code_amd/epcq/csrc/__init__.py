@@ -1,0 +1,502 @@
+"""`epcq.csrc`: the op module of the epcq path, backed by libepcq_hip.so.
+
+Drop-in replacement of the reference's pybind11 extension `epcq.csrc`
+(epcq/csrc/epcq.cpp:5-55): the same four ops (`scatter`, `scatter_hashtable`,
+`quick_sampling`, `quick_query_for_nearby_point`) with the same tensor
+signatures, return tuples and error texts, and the same four spec classes with
+read/write attributes.  It adds `build_hashtable` (fused pts2plane),
+`knn_sampling` (compact K-nearest sampling with ray depths, distances and IDW
+features), `idw_features`, and `max_live_samples`.
+
+Each op checks its tensors on the host, allocates its outputs with torch on
+the input's device, and calls the C ABI (include/epcq_abi.h) through ctypes
+on the current HIP stream of that device.  The ABI never allocates device
+memory and never synchronises.  There is no CPU fallback: if the HIP library
+is missing this module fails to import.
+"""
+import ctypes
+import os
+
+import torch
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libepcq_hip.so")
+
+MODE_COMPAT = 0
+MODE_KNN = 1
+_MODES = {"compat": MODE_COMPAT, "knn": MODE_KNN, MODE_COMPAT: MODE_COMPAT, MODE_KNN: MODE_KNN}
+
+
+class _Camera(ctypes.Structure):
+    _fields_ = [
+        ("c2w", ctypes.c_void_p),
+        ("w2c", ctypes.c_void_p),
+        ("fx", ctypes.c_float),
+        ("fy", ctypes.c_float),
+        ("cx", ctypes.c_float),
+        ("cy", ctypes.c_float),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+    ]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("kernel_size", ctypes.c_int32),
+        ("t_intvl", ctypes.c_float),
+        ("radius_threshold", ctypes.c_float),
+        ("sigma_threshold", ctypes.c_float),
+        ("background_brightness", ctypes.c_float),
+        ("stop_threshold", ctypes.c_float),
+        ("num_sample_points_per_ray", ctypes.c_int32),
+        ("num_point_cloud_per_sp", ctypes.c_int32),
+        ("num_point_per_ray", ctypes.c_int32),
+        ("sdf2weight_gaussian_alpha", ctypes.c_float),
+        ("sdf2weight_gaussian_gamma", ctypes.c_float),
+        ("topK", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_F32 = ctypes.c_float
+_SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes); the declarations of include/epcq_abi.h
+ABI_SYMBOLS = {
+    "epcq_abi_version": (_I32, []),
+    "epcq_last_error": (ctypes.c_char_p, []),
+    "epcq_scatter_hashtable": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "epcq_scatter": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P]),
+    "epcq_build_hashtable_workspace_size": (_SZ, [_I64, _I32, _I32]),
+    "epcq_build_hashtable": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "epcq_quick_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
+                                   _I32, _P, _P, _P, _P, _P]),
+    "epcq_quick_query": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
+                                _I64, _P, _P]),
+    "epcq_knn_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
+                                 _P, _I64, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P]),
+    "epcq_max_live_samples": (_I32, [ctypes.POINTER(_Options), _F32]),
+    "epcq_idw_features": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
+    "epcq_debug_expf": (_I32, [_P, _P, _I64, _P]),
+}
+
+
+def load_library(path=_LIB_PATH):
+    """Loads libepcq_hip.so and declares every ABI symbol; raises if absent."""
+    if not os.path.exists(path):
+        raise ImportError(
+            f"epcq.csrc: HIP library not found at {path}; build it with "
+            "`python __graft_entry__.py` or `python hashpoint-code_amd/build_lib.py`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in ABI_SYMBOLS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = load_library()
+LIBRARY_PATH = _LIB_PATH
+
+
+# ---------------------------------------------------------------- spec classes
+class NeuralPointsSpec:
+    """NeuralPointsSpec (epcq/csrc/include/data_spec.hpp:8-13)."""
+
+    def __init__(self):
+        self.xyz_data = None
+
+    def check(self):
+        _check_input(self.xyz_data, "xyz_data")
+
+
+class CameraSpec:
+    """CameraSpec (epcq/csrc/include/data_spec.hpp:15-31)."""
+
+    def __init__(self):
+        self.c2w = None
+        self.w2c = None
+        self.fx = 0.0
+        self.fy = 0.0
+        self.cx = 0.0
+        self.cy = 0.0
+        self.width = 0
+        self.height = 0
+
+    def check(self):
+        _check_input(self.c2w, "c2w")
+        _check_input(self.w2c, "w2c")
+        if not self.c2w.is_floating_point():
+            raise RuntimeError("Expected c2w.is_floating_point() to be true, but got false.")
+        if self.c2w.dim() != 2:
+            raise RuntimeError("Expected c2w.ndimension() == 2 to be true, but got false.")
+        if self.c2w.size(1) != 4:
+            raise RuntimeError("Expected c2w.size(1) == 4 to be true, but got false.")
+
+
+class RenderOptions:
+    """RenderOptions (epcq/csrc/include/data_spec.hpp:49-69)."""
+
+    _FIELDS = [name for name, _ in _Options._fields_]
+
+    def __init__(self):
+        for name, ctype in _Options._fields_:
+            setattr(self, name, 0 if ctype is ctypes.c_int32 else 0.0)
+
+
+class NeuralPointsGrads:
+    """NeuralPointsGrads (epcq/csrc/include/data_spec.hpp:72-81)."""
+
+    def __init__(self):
+        self.grad_sigma_out = None
+        self.grad_sh_out = None
+        self.mask_out = None
+
+    def check(self):
+        _check_input(self.grad_sigma_out, "grad_sigma_out")
+        _check_input(self.grad_sh_out, "grad_sh_out")
+        _check_input(self.mask_out, "mask_out")
+
+
+# ---------------------------------------------------------------- helpers
+def _check_input(x, name):
+    """CHECK_INPUT (epcq/csrc/include/util.hpp:4-11)."""
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not x.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+_DTYPE_NAME = {torch.int32: "Int", torch.float32: "Float", torch.int64: "Long", torch.bool: "Bool",
+               torch.float64: "Double", torch.bfloat16: "BFloat16", torch.float16: "Half",
+               torch.uint8: "Byte", torch.int16: "Short", torch.int8: "Char"}
+
+
+def _check_dtype(x, dtype, name):
+    if x.dtype != dtype:
+        raise RuntimeError(
+            f"{name}: expected scalar type {_DTYPE_NAME.get(dtype, dtype)} but found "
+            f"{_DTYPE_NAME.get(x.dtype, x.dtype)}")
+
+
+def _check_same_device(device, *tensors):
+    for name, t in tensors:
+        if t is not None and t.device != device:
+            raise RuntimeError(f"{name} must be on {device}, found {t.device}")
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _raise_on(rc):
+    if rc != 0:
+        msg = _lib.epcq_last_error()
+        raise RuntimeError(msg.decode() if msg else f"epcq error {rc}")
+
+
+def _img_size(img_size):
+    h, w = img_size
+    return int(h), int(w)
+
+
+def _camera_struct(cam):
+    return _Camera(_ptr(cam.c2w), _ptr(cam.w2c), float(cam.fx), float(cam.fy), float(cam.cx),
+                   float(cam.cy), int(cam.width), int(cam.height))
+
+
+def _options_struct(opt):
+    vals = []
+    for name, ctype in _Options._fields_:
+        v = getattr(opt, name)
+        vals.append(int(v) if ctype is ctypes.c_int32 else float(v))
+    return _Options(*vals)
+
+
+def _hashtable_view(hashtable, cam_h, cam_w):
+    """The [P,2] / [H,W,2] hashtable check of epcq_render_topk.cu:351-357."""
+    if hashtable.dim() == 2:
+        hashtable = hashtable.view(cam_h, cam_w, 2)
+    if hashtable.dim() != 3 or hashtable.size(0) != cam_h or hashtable.size(1) != cam_w \
+            or hashtable.size(2) != 2:
+        dims = list(hashtable.shape) + [0, 0, 0]
+        print(f"hashtable size: {dims[0]}, {dims[1]}, {dims[2]}")
+        raise RuntimeError("hashtable size error")
+    return hashtable
+
+
+def _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable):
+    neural_points.check()
+    cam.check()
+    _check_input(sorted_point_idx_list, "sorted_point_idx_list")
+    _check_input(hashtable, "hashtable")
+    _check_dtype(neural_points.xyz_data, torch.float32, "xyz_data")
+    if neural_points.xyz_data.dim() != 2 or neural_points.xyz_data.size(1) != 3:
+        raise RuntimeError("xyz_data must have shape [N, 3]")
+    _check_dtype(cam.c2w, torch.float32, "c2w")
+    _check_dtype(cam.w2c, torch.float32, "w2c")
+    if cam.c2w.size(0) < 3 or cam.w2c.dim() != 2 or cam.w2c.size(0) < 3 or cam.w2c.size(1) != 4:
+        raise RuntimeError("c2w and w2c must have at least 3 rows of 4 values")
+    _check_dtype(sorted_point_idx_list, torch.int32, "sorted_point_idx_list")
+    _check_dtype(hashtable, torch.int32, "hashtable")
+    device = neural_points.xyz_data.device
+    _check_same_device(device, ("c2w", cam.c2w), ("w2c", cam.w2c),
+                       ("sorted_point_idx_list", sorted_point_idx_list), ("hashtable", hashtable))
+    return device
+
+
+# ---------------------------------------------------------------- ops
+def scatter_hashtable(point_idx_list, point2img_idx_list, depth_list, img_size, sort):
+    """scatter_hashtable (epcq/csrc/rasterization.cu:130-171).
+
+    Returns (sorted_point_idx_list int32 [N], sorted_depth_list float32 [N],
+    hash_table int32 [H*W, 2])."""
+    _check_input(point_idx_list, "point_idx_list")
+    _check_input(point2img_idx_list, "point2img_idx_list")
+    _check_input(depth_list, "depth_list")
+    _check_dtype(point_idx_list, torch.int32, "point_idx_list")
+    _check_dtype(point2img_idx_list, torch.int32, "point2img_idx_list")
+    _check_dtype(depth_list, torch.float32, "depth_list")
+    n = point_idx_list.numel()
+    if point2img_idx_list.numel() != n or depth_list.numel() != n:
+        raise RuntimeError("point_idx_list, point2img_idx_list and depth_list must have the same length")
+    H, W = _img_size(img_size)
+    dev = point_idx_list.device
+    _check_same_device(dev, ("point2img_idx_list", point2img_idx_list), ("depth_list", depth_list))
+    out_idx = torch.empty(n, dtype=torch.int32, device=dev)
+    out_depth = torch.empty(n, dtype=torch.float32, device=dev)
+    out_hash = torch.empty((H * W, 2), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_scatter_hashtable(_ptr(point_idx_list), _ptr(point2img_idx_list), _ptr(depth_list),
+                                              n, H, W, int(bool(sort)), _ptr(out_idx), _ptr(out_depth),
+                                              _ptr(out_hash), _stream(dev)))
+    return out_idx, out_depth, out_hash
+
+
+def scatter(point_idx_list, point2img_idx_list, depth_list, img_size, bins):
+    """scatter (epcq/csrc/rasterization.cu:88-128).
+
+    Returns (buffer_depth float32 [H*W, bins], buffer_point_index int32
+    [H*W, bins]); per pixel the `bins` smallest depths ascending, -1 padded."""
+    _check_input(point_idx_list, "point_idx_list")
+    _check_input(point2img_idx_list, "point2img_idx_list")
+    _check_input(depth_list, "depth_list")
+    _check_dtype(point_idx_list, torch.int32, "point_idx_list")
+    _check_dtype(point2img_idx_list, torch.int32, "point2img_idx_list")
+    _check_dtype(depth_list, torch.float32, "depth_list")
+    n = point_idx_list.numel()
+    if point2img_idx_list.numel() != n or depth_list.numel() != n:
+        raise RuntimeError("point_idx_list, point2img_idx_list and depth_list must have the same length")
+    H, W = _img_size(img_size)
+    bins = int(bins)
+    dev = point_idx_list.device
+    _check_same_device(dev, ("point2img_idx_list", point2img_idx_list), ("depth_list", depth_list))
+    out_depth = torch.empty((H * W, bins), dtype=torch.float32, device=dev)
+    out_idx = torch.empty((H * W, bins), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_scatter(_ptr(point_idx_list), _ptr(point2img_idx_list), _ptr(depth_list), n, H, W,
+                                    bins, _ptr(out_depth), _ptr(out_idx), _stream(dev)))
+    return out_depth, out_idx
+
+
+def build_hashtable(xyz, w2c, K, img_size, sort=False, workspace=None):
+    """Fused NeuralPoints.pts2plane (epcq/epcq.py:279-295).
+
+    xyz float32 [N,3]; w2c float32 [>=3, 4]; K float32 [3,3].  Returns
+    (sorted_point_idx_list int32 [N], sorted_depth_list float32 [N],
+    hash_table int32 [H*W, 2]) with pixel lists ordered by (pixel, index), or
+    (pixel, depth, index) when `sort`.  `workspace` is an optional uint8 CUDA
+    tensor of at least build_hashtable_workspace_size(N, H, W) bytes."""
+    _check_input(xyz, "xyz_data")
+    _check_input(w2c, "w2c")
+    _check_input(K, "K")
+    _check_dtype(xyz, torch.float32, "xyz_data")
+    _check_dtype(w2c, torch.float32, "w2c")
+    _check_dtype(K, torch.float32, "K")
+    if xyz.dim() != 2 or xyz.size(1) != 3:
+        raise RuntimeError("xyz_data must have shape [N, 3]")
+    if w2c.dim() != 2 or w2c.size(0) < 3 or w2c.size(1) != 4:
+        raise RuntimeError("w2c must have at least 3 rows of 4 values")
+    if K.dim() != 2 or K.size(0) != 3 or K.size(1) != 3:
+        raise RuntimeError("K must have shape [3, 3]")
+    H, W = _img_size(img_size)
+    n = xyz.size(0)
+    dev = xyz.device
+    _check_same_device(dev, ("w2c", w2c), ("K", K))
+    need = build_hashtable_workspace_size(n, H, W)
+    if workspace is None:
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    elif workspace.numel() * workspace.element_size() < need:
+        raise RuntimeError("build_hashtable: workspace too small")
+    out_idx = torch.empty(n, dtype=torch.int32, device=dev)
+    out_depth = torch.empty(n, dtype=torch.float32, device=dev)
+    out_hash = torch.empty((H * W, 2), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_build_hashtable(_ptr(xyz), n, _ptr(w2c), _ptr(K), H, W, int(bool(sort)),
+                                            _ptr(out_idx), _ptr(out_depth), _ptr(out_hash), _ptr(workspace),
+                                            workspace.numel() * workspace.element_size(), _stream(dev)))
+    return out_idx, out_depth, out_hash
+
+
+def build_hashtable_workspace_size(n, height, width):
+    return int(_lib.epcq_build_hashtable_workspace_size(int(n), int(height), int(width)))
+
+
+def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, mode="compat"):
+    """quick_sampling (epcq/csrc/epcq_render_topk.cu:341-417).
+
+    Returns (output_sp_xyz float32 [H,W,S,3], output_pc_idx int32 [H,W,S,Kc],
+    output_sp_t float32 [H,W,S], mask_out bool [H,W,S]).  `mode` "compat"
+    keeps the reference's neighbour buffer; "knn" returns the Kc nearest by
+    (distance, index)."""
+    device = _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable)
+    H, W = int(cam.height), int(cam.width)
+    hashtable = _hashtable_view(hashtable, H, W)
+    S = int(options.num_sample_points_per_ray)
+    Kc = int(options.num_point_cloud_per_sp)
+    xyz = neural_points.xyz_data
+    sp_xyz = torch.empty((H, W, S, 3), dtype=torch.float32, device=device)
+    sp_t = torch.empty((H, W, S), dtype=torch.float32, device=device)
+    pc_idx = torch.empty((H, W, S, Kc), dtype=torch.int32, device=device)
+    mask = torch.empty((H, W, S), dtype=torch.bool, device=device)
+    camera, opts = _camera_struct(cam), _options_struct(options)
+    with torch.cuda.device(device):
+        _raise_on(_lib.epcq_quick_sampling(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
+                                           _ptr(sorted_point_idx_list), _ptr(hashtable), _MODES[mode],
+                                           _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _stream(device)))
+    return sp_xyz, pc_idx, sp_t, mask
+
+
+def quick_query_for_nearby_point(neural_points, cam, options, sorted_point_idx_list, hashtable, indices_of_ray):
+    """quick_query_for_nearby_point (epcq/csrc/epcq_render_topk.cu:287-339).
+
+    Returns int32 [R, num_point_per_ray]: the hits of each ray's neighbour
+    buckets in lane order, -1 padded."""
+    device = _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable)
+    _check_input(indices_of_ray, "indices_of_ray")
+    _check_dtype(indices_of_ray, torch.int32, "indices_of_ray")
+    _check_same_device(device, ("indices_of_ray", indices_of_ray))
+    H, W = int(cam.height), int(cam.width)
+    hashtable = _hashtable_view(hashtable, H, W)
+    R = indices_of_ray.numel()
+    cap = int(options.num_point_per_ray)
+    out = torch.empty((R, cap), dtype=torch.int32, device=device)
+    xyz = neural_points.xyz_data
+    camera, opts = _camera_struct(cam), _options_struct(options)
+    with torch.cuda.device(device):
+        _raise_on(_lib.epcq_quick_query(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
+                                        _ptr(sorted_point_idx_list), _ptr(hashtable), _ptr(indices_of_ray), R,
+                                        _ptr(out), _stream(device)))
+    return out
+
+
+def max_live_samples(options, r_max):
+    """Bound on the samples any ray can emit under these options when every
+    point radius is below r_max."""
+    return int(_lib.epcq_max_live_samples(ctypes.byref(_options_struct(options)), float(r_max)))
+
+
+def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, rays=None, radius=None,
+                 mode="knn", s_max=None, feat=None):
+    """Compact K-nearest sampling.
+
+    Same samples as quick_sampling, stored per ray.  Returns a dict with
+    n_samples int32 [R], sample_t float32 [R,s_max], nb_idx int32
+    [R,s_max,Kc], and for mode "knn" nb_t / nb_dist float32 [R,s_max,Kc]
+    (ray depth and point-to-ray distance of each neighbour); with `feat`
+    (float32 or bfloat16 [N,F]) also w float32 [R,s_max,Kc] and agg float32
+    [R,s_max,F] (inverse-distance weights and weighted feature sum).
+    `rays` int32 pixel ids (default: all pixels); `radius` float32 [N]
+    per-point radius (default: options.radius_threshold)."""
+    device = _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable)
+    H, W = int(cam.height), int(cam.width)
+    hashtable = _hashtable_view(hashtable, H, W)
+    xyz = neural_points.xyz_data
+    if rays is not None:
+        _check_input(rays, "rays")
+        _check_dtype(rays, torch.int32, "rays")
+        _check_same_device(device, ("rays", rays))
+        R = rays.numel()
+    else:
+        R = H * W
+    if radius is not None:
+        _check_input(radius, "radius")
+        _check_dtype(radius, torch.float32, "radius")
+        _check_same_device(device, ("radius", radius))
+        if radius.numel() != xyz.size(0):
+            raise RuntimeError("radius must have one entry per point")
+        r_max = float(radius.max().item()) if radius.numel() else 0.0
+    else:
+        r_max = float(options.radius_threshold)
+    if s_max is None:
+        s_max = max(1, max_live_samples(options, r_max))
+    Kc = int(options.num_point_cloud_per_sp)
+    knn = _MODES[mode] == MODE_KNN
+    out = {
+        "n_samples": torch.empty(R, dtype=torch.int32, device=device),
+        "sample_t": torch.empty((R, s_max), dtype=torch.float32, device=device),
+        "nb_idx": torch.empty((R, s_max, Kc), dtype=torch.int32, device=device),
+    }
+    if knn:
+        out["nb_t"] = torch.empty((R, s_max, Kc), dtype=torch.float32, device=device)
+        out["nb_dist"] = torch.empty((R, s_max, Kc), dtype=torch.float32, device=device)
+    F = 0
+    bf16 = 0
+    if feat is not None:
+        _check_input(feat, "feat")
+        _check_same_device(device, ("feat", feat))
+        if feat.dtype not in (torch.float32, torch.bfloat16):
+            raise RuntimeError("feat must be float32 or bfloat16")
+        if feat.dim() != 2 or feat.size(0) != xyz.size(0):
+            raise RuntimeError("feat must have shape [N, F]")
+        F = feat.size(1)
+        bf16 = int(feat.dtype == torch.bfloat16)
+        out["w"] = torch.empty((R, s_max, Kc), dtype=torch.float32, device=device)
+        out["agg"] = torch.empty((R, s_max, F), dtype=torch.float32, device=device)
+    camera, opts = _camera_struct(cam), _options_struct(options)
+    with torch.cuda.device(device):
+        _raise_on(_lib.epcq_knn_sampling(
+            _ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts), _ptr(sorted_point_idx_list),
+            _ptr(hashtable), _ptr(radius), _ptr(rays), R, _MODES[mode], int(s_max), _ptr(out["n_samples"]),
+            _ptr(out["sample_t"]), _ptr(out["nb_idx"]), _ptr(out.get("nb_t")), _ptr(out.get("nb_dist")),
+            _ptr(feat), bf16, F, _ptr(out.get("w")), _ptr(out.get("agg")), _stream(device)))
+    return out
+
+
+def idw_features(xyz, feat, sp_xyz, nb_idx):
+    """IDW weights and weighted features of dense samples
+    (epcq/epcq.py:434-437,449-452).  sp_xyz [M,3], nb_idx int32 [M,K] ->
+    (w float32 [M,K], agg float32 [M,F])."""
+    for name, t in (("xyz_data", xyz), ("feat", feat), ("sp_xyz", sp_xyz), ("nb_idx", nb_idx)):
+        _check_input(t, name)
+    _check_dtype(xyz, torch.float32, "xyz_data")
+    _check_dtype(sp_xyz, torch.float32, "sp_xyz")
+    _check_dtype(nb_idx, torch.int32, "nb_idx")
+    if feat.dtype not in (torch.float32, torch.bfloat16):
+        raise RuntimeError("feat must be float32 or bfloat16")
+    dev = xyz.device
+    _check_same_device(dev, ("feat", feat), ("sp_xyz", sp_xyz), ("nb_idx", nb_idx))
+    M = sp_xyz.numel() // 3
+    K = nb_idx.size(-1) if nb_idx.dim() else 0
+    F = feat.size(1)
+    w = torch.empty((M, K), dtype=torch.float32, device=dev)
+    agg = torch.empty((M, F), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_idw_features(_ptr(xyz), _ptr(feat), int(feat.dtype == torch.bfloat16), F,
+                                         _ptr(sp_xyz), _ptr(nb_idx), M, K, _ptr(w), _ptr(agg), _stream(dev)))
+    return w, agg
+
+
+def debug_expf(x):
+    """exp() as evaluated by the device kernels (test hook)."""
+    _check_input(x, "x")
+    _check_dtype(x, torch.float32, "x")
+    out = torch.empty_like(x)
+    with torch.cuda.device(x.device):
+        _raise_on(_lib.epcq_debug_expf(_ptr(x), _ptr(out), x.numel(), _stream(x.device)))
+    return out

@@ -1,0 +1,260 @@
+"""Python API of the epcq searching + sampling path (mirrors epcq/epcq.py of
+the reference: CameraSpec :15-63, RenderOptions :81-114, NeuralPointsGrads
+:133-144, NeuralPoints :147-327, FeatureQuery :329-387, calculate_weight
+:434-437).
+
+NeuralPoints binds the ops of `epcq.csrc` exactly as the reference does
+(`scatter`, `scatter_hashtable`, `quick_sampling`); `pts2plane` runs the fused
+gfx950 build (projection, pixel keys, stable sort and hash table in one
+pipeline) instead of torch matmuls + torch.sort + scatter_hashtable, with
+pixel lists in the deterministic (pixel, index) order.  The rendering head of
+the reference module (positional encoding, the Aggregator MLP, clamp_t,
+render) comes after this path and is not part of this package.
+"""
+from dataclasses import dataclass
+from typing import Optional, Union
+
+import torch
+import torch.nn as nn
+
+from . import utils
+
+_C = utils._get_c_extension()
+
+
+@dataclass
+class CameraSpec:
+    c2w: torch.Tensor
+    w2c: torch.Tensor
+    K: torch.Tensor
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    width: int
+    height: int
+
+    def _to_cpp(self):
+        spec = _C.CameraSpec()
+        for name in ("c2w", "w2c", "fx", "fy", "cx", "cy", "width", "height"):
+            setattr(spec, name, getattr(self, name))
+        return spec
+
+    @staticmethod
+    def decomposeK(K: torch.Tensor):
+        """(fx, fy, cx, cy) of an intrinsics matrix."""
+        return K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+
+    def pixelidx2hwidx(self, pixelidx):
+        return pixelidx // self.width, pixelidx % self.width
+
+    def hwidx2pixelidx(self, h_idx, w_idx):
+        return h_idx * self.width + w_idx
+
+
+@dataclass
+class RenderOptions:
+    kernel_size: int = 5
+    t_intvl: float = 0.01
+    radius_threshold: float = 0.016
+    sigma_threshold: float = 1e-4
+    background_brightness: float = 1.0
+    stop_threshold: float = 0.001
+    num_sample_points_per_ray: int = 16
+    num_point_cloud_per_sp: int = 4
+    num_point_per_ray: int = 256
+    sdf2weight_gaussian_alpha: float = 0.2
+    sdf2weight_gaussian_gamma: float = 0.9
+    sparsity_loss_w: float = 5e-4
+    topK: int = 8
+
+    def _to_cpp(self):
+        opt = _C.RenderOptions()
+        for name in _C.RenderOptions._FIELDS:
+            setattr(opt, name, getattr(self, name))
+        return opt
+
+
+@dataclass
+class NeuralPointsGrads:
+    grad_sigma: torch.Tensor
+    grad_sh: torch.Tensor
+    mask_out: torch.Tensor
+
+    def _to_cpp(self):
+        grads = _C.NeuralPointsGrads()
+        grads.grad_sigma_out = self.grad_sigma
+        grads.grad_sh_out = self.grad_sh
+        grads.mask_out = self.mask_out
+        return grads
+
+
+class NeuralPoints(nn.Module):
+    """A point cloud with its per-point attributes, and the search ops.
+
+    pc_type 'pc' wraps the given `points` [N,3]; 'pointnerf' loads a Point-NeRF
+    checkpoint (tensors only, torch.load(weights_only=True)); 'gt_pc' reads a
+    point cloud file with open3d when that package is installed."""
+
+    def __init__(self,
+                 basis_dim: int = 9,
+                 device: Union[torch.device, str] = "cuda:0",
+                 pc_type: str = "pointnerf",
+                 gt_pc_path: Optional[str] = None,
+                 pointnerf_path: str = "pointnerf_ckpt/nerf_synthesis/lego.pth",
+                 alpha: float = 5.0,
+                 points=None):
+        super().__init__()
+        if pc_type not in ("pointnerf", "gt_pc", "pc"):
+            raise AssertionError(f"unknown pc_type {pc_type!r}")
+        self.pc_type = pc_type
+        if pc_type == "pointnerf":
+            if not str(pointnerf_path).endswith(".pth"):
+                raise ValueError("Unsupported pointnerf_path: {}".format(pointnerf_path))
+            ckpt = torch.load(pointnerf_path, map_location="cpu", weights_only=True)
+            xyz = ckpt["neural_points.xyz"]
+            attrs = {
+                "rgb_data": ckpt["neural_points.points_color"].squeeze(0),
+                "dir_data": ckpt["neural_points.points_dir"].squeeze(0),
+                "density_data": ckpt["neural_points.points_conf"].squeeze(0),
+                "feat_data": ckpt["neural_points.points_embeding"].squeeze(0),
+            }
+            self.xyz_data = nn.Parameter(xyz, requires_grad=False)
+            for name, value in attrs.items():
+                setattr(self, name, nn.Parameter(value, requires_grad=True))
+        elif pc_type == "gt_pc":
+            if gt_pc_path is None:
+                raise ValueError("gt_pc_path is required")
+            import numpy as np
+            try:
+                import open3d as o3d
+            except ImportError as e:
+                raise ImportError("pc_type='gt_pc' needs open3d to read " + str(gt_pc_path)) from e
+            pcd = o3d.io.read_point_cloud(gt_pc_path)
+            pts = torch.tensor(np.asarray(pcd.points), dtype=torch.float32, device=device)
+            cols = torch.tensor(np.asarray(pcd.colors), dtype=torch.float32, device=device)
+            self.basis_dim = basis_dim
+            self.xyz_data = nn.Parameter(pts, requires_grad=False)
+            self.rgb_data = nn.Parameter(cols, requires_grad=True)
+            self.dir_data = nn.Parameter(torch.zeros_like(pts), requires_grad=True)
+            self.density_data = nn.Parameter(torch.zeros(pts.shape[0], 1, device=device), requires_grad=True)
+            self.feat_data = nn.Parameter(torch.zeros(pts.shape[0], 32, device=device), requires_grad=True)
+        else:
+            if points is None:
+                raise AssertionError("pc_type='pc' needs points")
+            self.xyz_data = nn.Parameter(points, requires_grad=False)
+            self.rgb_data = nn.Parameter(torch.zeros_like(points), requires_grad=False)
+            self.dir_data = nn.Parameter(torch.zeros_like(points), requires_grad=False)
+            self.density_data = nn.Parameter(torch.zeros((points.shape[0], 1)), requires_grad=False)
+            self.feat_data = nn.Parameter(torch.zeros_like(points), requires_grad=False)
+        self.capacity = self.xyz_data.shape[0]
+        self.multi_bins_rasterize = _C.__dict__["scatter"]
+        self.rasterize_hashtable = _C.__dict__["scatter_hashtable"]
+        self.quick_sampling = _C.__dict__["quick_sampling"]
+
+    def _to_cpp(self):
+        spec = _C.NeuralPointsSpec()
+        spec.xyz_data = self.xyz_data
+        return spec
+
+    # -- torch forms of the projection (epcq.py:251-277); the fused build in
+    #    pts2plane evaluates the same formulas in a fixed operation order
+    def _world2cam(self, cam: CameraSpec, pc_xyz: torch.Tensor = None):
+        xyz = self.xyz_data if pc_xyz is None else pc_xyz
+        pts_c = (cam.w2c[:3, :3] @ xyz.t() + cam.w2c[:3, 3:4]).t()
+        return pts_c, pts_c[:, 2:3]
+
+    def _cam2image(self, pts_c: torch.Tensor, cam: CameraSpec):
+        scr = cam.K @ pts_c.t()
+        scr = scr / scr[2:3, :]
+        return scr[0], scr[1], scr[0].to(torch.int32), scr[1].to(torch.int32)
+
+    def _convetpixel(self, screen_x, screen_y, hw):
+        key = screen_x + screen_y * hw[1]
+        inside = (screen_x >= 0) & (screen_x < hw[1]) & (screen_y >= 0) & (screen_y < hw[0])
+        return torch.where(inside, key, torch.full_like(key, hw[0] * hw[1]))
+
+    def _uv2ray_index(self, uv, width):
+        return (uv[:, 1] * width + uv[:, 0]).to(torch.int32)
+
+    def pts2plane(self, cam: CameraSpec, opt: RenderOptions, pc_xyz: torch.Tensor = None, sort: bool = False,
+                  workspace: Optional[torch.Tensor] = None):
+        """Per-pixel point lists of the view: (sorted_point_idx_list int32
+        [N], sorted_depth_list float32 [N], hash_table int32 [H*W, 2])."""
+        xyz = self.xyz_data if pc_xyz is None else pc_xyz
+        dev = xyz.device
+        w2c = cam.w2c.to(device=dev, dtype=torch.float32).contiguous()
+        K = cam.K.to(device=dev, dtype=torch.float32).contiguous()
+        return _C.build_hashtable(xyz.detach(), w2c, K, [int(cam.height), int(cam.width)], bool(sort),
+                                  workspace=workspace)
+
+    def render_image_hashtable_topk(self, cam: CameraSpec,
+                                    opt: RenderOptions,
+                                    sorted_point_idx_list: torch.Tensor,
+                                    sorted_depth_list: torch.Tensor,
+                                    hashtable: torch.Tensor,
+                                    save_variables_for_debug: bool = False,
+                                    mode: str = "compat"):
+        """(output_sp_xyz, output_pc_idx, output_sp_t, mask_out) of every
+        pixel; `save_variables_for_debug` is accepted and ignored."""
+        return self.quick_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list,
+                                   hashtable, mode=mode)
+
+    def knn_sampling(self, cam: CameraSpec, opt: RenderOptions, sorted_point_idx_list: torch.Tensor,
+                     hashtable: torch.Tensor, rays: Optional[torch.Tensor] = None,
+                     radius: Optional[torch.Tensor] = None, mode: str = "knn", feat: Optional[torch.Tensor] = None,
+                     s_max: Optional[int] = None):
+        """Compact K-nearest sampling (see epcq.csrc.knn_sampling)."""
+        return _C.knn_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list, hashtable,
+                               rays=rays, radius=radius, mode=mode, s_max=s_max, feat=feat)
+
+
+class FeatureQuery(nn.Module):
+    """Gathers the attributes of each live sample's neighbours
+    (epcq.py:329-387)."""
+
+    def generate_rays(self, cam: CameraSpec):
+        """Unnormalised pixel-corner ray directions [H,W,3] and origins."""
+        dev = cam.c2w.device
+        W, H = int(cam.width), int(cam.height)
+        x, y = torch.meshgrid(torch.arange(W, dtype=torch.float32, device=dev),
+                              torch.arange(H, dtype=torch.float32, device=dev), indexing="xy")
+        cx = (x - cam.cx) / cam.fx
+        cy = (y - cam.cy) / cam.fy
+        R = cam.c2w[:3, :3]
+        rays_d = torch.stack([R[i, 0] * cx + R[i, 1] * cy + R[i, 2] for i in range(3)], dim=-1)
+        rays_o = cam.c2w[:3, 3].expand_as(rays_d)
+        return rays_o, rays_d
+
+    def forward(self, neuralpoints: NeuralPoints, output_sp_xyz: torch.Tensor, output_pc_idx: torch.Tensor,
+                mask_out: torch.Tensor, cam: CameraSpec, ray_index_uv: torch.Tensor = None):
+        S, Kc = output_pc_idx.shape[-2:]
+        live = mask_out.reshape(-1)
+        sp = output_sp_xyz.reshape(-1, 3)[live]
+        idx = output_pc_idx.reshape(-1, Kc)[live].long()
+        valid = idx != -1
+        safe = idx.clamp_min(0)
+
+        def gather(attr):
+            out = attr[safe]
+            return torch.where(valid.unsqueeze(-1), out, torch.zeros((), dtype=out.dtype, device=out.device))
+
+        pc_xyz = gather(neuralpoints.xyz_data)
+        pc_rgb = gather(neuralpoints.rgb_data)
+        pc_dir = gather(neuralpoints.dir_data)
+        pc_feat = gather(neuralpoints.feat_data)
+        _, rays_d = self.generate_rays(cam)
+        if ray_index_uv is not None and len(ray_index_uv) != 0:
+            rays_d = rays_d[ray_index_uv[:, 1], ray_index_uv[:, 0]]
+            rays_d = rays_d[:, None, :].expand(-1, S, -1).reshape(-1, 3)
+        else:
+            rays_d = rays_d[:, :, None, :].expand(-1, -1, S, -1).reshape(-1, 3)
+        return pc_xyz, pc_rgb, pc_dir, pc_feat, sp, valid, rays_d[live]
+
+
+def calculate_weight(dist: torch.Tensor, mask: torch.Tensor = None):
+    """Inverse-distance weights (Aggregator.calculate_weight, epcq.py:434-437)."""
+    inv = 1.0 / (dist + 1e-6)
+    if mask is not None:
+        inv = inv * mask.float()
+    return inv / (inv.sum(dim=-1, keepdim=True) + 1e-6)

@@ -1,0 +1,173 @@
+/*
+ * epcq_abi.h -- C ABI of the MI355X (gfx950) epcq searching + sampling path.
+ *
+ * This is the boundary the Python module `epcq.csrc` binds with ctypes.  It
+ * replaces the pybind11 extension `epcq.csrc` of the reference
+ * (epcq/csrc/epcq.cpp:5-55, setup.py:37-46).  Every entry point:
+ *   - takes plain device pointers and sizes (no torch types);
+ *   - never allocates device memory: outputs and workspace are supplied by the
+ *     caller, so calls can be captured into a HIP graph;
+ *   - enqueues its kernels on `stream` (a hipStream_t; NULL = the legacy
+ *     default stream) and returns without synchronising;
+ *   - returns EPCQ_OK or an error code; epcq_last_error() gives the message
+ *     of the last failure on the calling thread (launch errors included).
+ * Integers are int32 and floats IEEE binary32 unless stated otherwise.
+ */
+#ifndef EPCQ_ABI_H
+#define EPCQ_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPCQ_ABI_VERSION 1
+
+enum epcq_status {
+    EPCQ_OK = 0,
+    EPCQ_ERR_INVALID = 1,   /* bad argument (shape, size, option) */
+    EPCQ_ERR_HIP = 2,       /* a HIP runtime / launch error */
+    EPCQ_ERR_WORKSPACE = 3  /* workspace too small */
+};
+
+/* neighbour selection around each sample point */
+enum epcq_mode {
+    EPCQ_MODE_COMPAT = 0,  /* the reference's 8-entry buffer (epcq_render_topk.cu:35-79) */
+    EPCQ_MODE_KNN = 1      /* K nearest by (distance, index), ascending */
+};
+
+/* CameraSpec (epcq/csrc/include/data_spec.hpp:15-31, epcq/epcq.py:15-37).
+ * c2w / w2c are device pointers to row-major matrices with >= 3 rows of 4
+ * floats (the reference's [>=3,4] CUDA tensors). */
+typedef struct epcq_camera {
+    const float* c2w;
+    const float* w2c;
+    float fx, fy, cx, cy;
+    int32_t width, height;
+} epcq_camera;
+
+/* RenderOptions (epcq/csrc/include/data_spec.hpp:49-69, epcq/epcq.py:81-114). */
+typedef struct epcq_options {
+    int32_t kernel_size;
+    float t_intvl;
+    float radius_threshold;
+    float sigma_threshold;
+    float background_brightness;
+    float stop_threshold;
+    int32_t num_sample_points_per_ray;
+    int32_t num_point_cloud_per_sp;
+    int32_t num_point_per_ray;
+    float sdf2weight_gaussian_alpha;
+    float sdf2weight_gaussian_gamma;
+    int32_t topK;
+} epcq_options;
+
+int epcq_abi_version(void);
+const char* epcq_last_error(void);
+
+/*
+ * scatter_hashtable -- replaces `scatter_hashtable`
+ * (epcq/csrc/epcq.cpp:6, epcq/csrc/rasterization.cu:130-171).
+ * Inputs are pixel-sorted (keys ascending, off-image key = H*W) as produced
+ * by epcq.py:283-286.  Writes every element of out_idx[n], out_depth[n]
+ * (-1 outside pixel runs) and out_hash[H*W*2] ((start, count) or (-1, -1));
+ * with sort != 0 each run is ordered by depth ascending, stable.
+ */
+int epcq_scatter_hashtable(const int32_t* point_idx, const int32_t* keys, const float* depth,
+                           int64_t n, int32_t height, int32_t width, int32_t sort,
+                           int32_t* out_idx, float* out_depth, int32_t* out_hash,
+                           void* stream);
+
+/*
+ * scatter -- replaces `scatter` (epcq/csrc/epcq.cpp:5,
+ * epcq/csrc/rasterization.cu:88-128).  Per pixel the `bins` smallest depths
+ * ascending: out_depth / out_idx [H*W, bins], -1 where unused.  Negative
+ * depths are skipped.
+ */
+int epcq_scatter(const int32_t* point_idx, const int32_t* keys, const float* depth, int64_t n,
+                 int32_t height, int32_t width, int32_t bins, float* out_depth, int32_t* out_idx,
+                 void* stream);
+
+/*
+ * build_hashtable -- fused replacement of NeuralPoints.pts2plane
+ * (epcq/epcq.py:279-295 = _world2cam, _cam2image, _convetpixel, torch.sort,
+ * scatter_hashtable).  xyz [n,3]; w2c >= 3 rows x 4 and K 3x3, row-major
+ * device pointers.  Output as epcq_scatter_hashtable, with pixel lists
+ * ordered by (pixel, point index), or (pixel, depth, point index) when
+ * sort != 0.  workspace: device memory of at least
+ * epcq_build_hashtable_workspace_size(n, height, width) bytes.
+ */
+size_t epcq_build_hashtable_workspace_size(int64_t n, int32_t height, int32_t width);
+int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2c, const float* K,
+                         int32_t height, int32_t width, int32_t sort,
+                         int32_t* out_idx, float* out_depth, int32_t* out_hash,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * quick_sampling -- replaces `quick_sampling` (epcq/csrc/epcq.cpp:7,
+ * epcq/csrc/epcq_render_topk.cu:341-417).  hash [H*W,2]; writes every element
+ * of sp_xyz [H,W,S,3], pc_idx [H,W,S,Kc], sp_t [H,W,S] and mask [H,W,S]
+ * (bool bytes), S = num_sample_points_per_ray, Kc = num_point_cloud_per_sp.
+ * For kernel_size > 5 (where the reference overflows its 32-entry neighbour
+ * list) all kernel_size^2 neighbour pixels are used in both stages.
+ */
+int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
+                        const epcq_options* opt, const int32_t* sorted_idx, const int32_t* hash,
+                        int32_t mode, float* sp_xyz, int32_t* pc_idx, float* sp_t, uint8_t* mask,
+                        void* stream);
+
+/*
+ * quick_query_for_nearby_point -- replaces `quick_query_for_nearby_point`
+ * (epcq/csrc/epcq.cpp:8, epcq/csrc/epcq_render_topk.cu:287-339).
+ * rays [R] pixel ids; out [R, num_point_per_ray], hits appended in
+ * neighbour-lane order, -1 padded.  Ray ids outside [0, H*W) give a -1 row.
+ */
+int epcq_quick_query(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
+                     const int32_t* sorted_idx, const int32_t* hash, const int32_t* rays,
+                     int64_t R, int32_t* out, void* stream);
+
+/*
+ * knn_sampling -- compact K-nearest sampling (no counterpart op in the
+ * reference; same samples as quick_sampling, epcq_render_topk.cu:82-220).
+ *   rays      [R] pixel ids, or NULL for all H*W pixels in order (R ignored);
+ *   radius    [n] per-point radius or NULL (= radius_threshold);
+ *   s_max     samples stored per ray (>= epcq_max_live_samples(opt, r_max));
+ * outputs (all per ray, s = 0..s_max-1, k = 0..Kc-1):
+ *   n_samples [R]            number of live samples;
+ *   sample_t  [R,s_max]      sample depth t (0 where not live);
+ *   nb_idx    [R,s_max,Kc]   neighbour point index (-1 where empty);
+ *   nb_t      [R,s_max,Kc]   neighbour ray depth (EPCQ_MODE_KNN, may be NULL);
+ *   nb_dist   [R,s_max,Kc]   neighbour point-to-ray distance (KNN, may be NULL);
+ * and, when feat != NULL (f32 [n,F] or bf16 [n,F] with feat_bf16 = 1), the
+ * inverse-distance weights of epcq/epcq.py:434-437,449-452:
+ *   w   [R,s_max,Kc]   weights;   agg [R,s_max,F]   sum_k w_k * feat[idx_k].
+ */
+int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
+                      const epcq_options* opt, const int32_t* sorted_idx, const int32_t* hash,
+                      const float* radius, const int32_t* rays, int64_t R, int32_t mode,
+                      int32_t s_max, int32_t* n_samples, float* sample_t, int32_t* nb_idx,
+                      float* nb_t, float* nb_dist, const void* feat, int32_t feat_bf16,
+                      int32_t F, float* w, float* agg, void* stream);
+
+/* Upper bound on live samples per ray for these options and radius bound. */
+int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
+
+/*
+ * idw_features -- FeatureQuery gathers + Aggregator.calculate_weight
+ * (epcq/epcq.py:345-387, 434-437, 449-452) on dense samples:
+ * sp_xyz [M,3], nb_idx [M,K] -> w [M,K], agg [M,F].
+ */
+int epcq_idw_features(const float* xyz, const void* feat, int32_t feat_bf16, int32_t F,
+                      const float* sp_xyz, const int32_t* nb_idx, int64_t M, int32_t K,
+                      float* w, float* agg, void* stream);
+
+/* Test hook: our exp() on the device, out[i] = exp(in[i]). */
+int epcq_debug_expf(const float* in, float* out, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPCQ_ABI_H */
