@@ -7,432 +7,141 @@
 //   epcq_quick_query     radius query for a ray list (:222-339)
 //   epcq_idw_features    IDW weights + weighted features (epcq/epcq.py:434-452)
 //
-// One lane owns one ray.  The 64 rays of a wave are 64 consecutive pixels of
-// a row, whose k x k windows overlap almost entirely, so the hash entries,
-// point indices and coordinates they gather are shared through L1/L2 and the
-// wave stays converged.  A ray's hits are never stored: the candidate walk is
-// repeated instead (tmin/tmax pass, slot pass, one pass per live sample), so
-// per-lane state stays in registers and nothing spills.
-//
-// Slot selection keeps a table of the 8 lowest occupied depth slots with
-// their distance sums, accumulated in the canonical (neighbour lane, bucket
-// position) order.  When more than 8 slots are occupied below the slot where
-// the front-to-back walk stops, the slot pass is repeated for the next
-// window of slots.  Results equal a full 128-slot histogram.
-#include <climits>
+// Full-image sampling runs one 256-lane block per tile of rays.  The block
+// first stages its tile's halo region -- the hash entries of the
+// (tw + k - 1) x (th + k - 1) neighbour pixels and, row segment by row
+// segment (contiguous in the row-major point lists), the coordinates and
+// indices of their points as float4 -- into LDS with coalesced loads; every
+// candidate walk of every ray then reads LDS.  A tile whose region holds more
+// points than the LDS capacity reads global memory instead.  Ray lists (no
+// tile structure) read global memory.  The per-ray logic is in
+// epcq_sampler.h.
+#include <cmath>
+#include <mutex>
 
 #include "epcq_common.h"
-#include "epcq_device.h"
+#include "epcq_sampler.h"
+
+extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
 
 namespace epcq {
+
+float squared_threshold(float r) {
+    if (!(r > 0.0f)) return 0.0f;  // no distance is below r (NaN r included)
+    if (std::isinf(r)) return INFINITY;
+    float s = r * r;
+    if (std::sqrt(s) >= r) {
+        while (s > 0.0f && std::sqrt(std::nextafter(s, 0.0f)) >= r) s = std::nextafter(s, 0.0f);
+    } else {
+        while (!(std::sqrt(s) >= r)) s = std::nextafter(s, INFINITY);
+    }
+    return s;
+}
+
+TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags) {
+    TilePlan p{};
+    p.tw = tile_w;
+    p.th = tile_h;
+    p.xcd = (flags & EPCQ_TUNE_XCD_ORDER) != 0;
+    p.rw_max = p.tw + k - 1;
+    p.rh_max = p.th + k - 1;
+    const size_t fixed = align_up((size_t)p.rw_max * p.rh_max * 8 + (size_t)(2 * p.rh_max + 1) * 4, 16);
+    const size_t limit = 64 * 1024;
+    size_t c = (size_t)std::max(cap, 0);
+    if (fixed + c * 16 > limit) c = fixed < limit ? (limit - fixed) / 16 : 0;
+    p.cap = (int)c;
+    p.lds_bytes = c * 16 + fixed;
+    return p;
+}
+
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kSlotTable = 8;
+constexpr int kMaxLiveSamples = 16;  // samples per ray the sampler can emit
 
-struct SampleArgs {
-    const float* xyz;
-    const int* sorted_idx;
-    const int2* hash;
-    const float* radius;  // per-point radius or nullptr
+std::mutex g_tuning_mu;
+epcq_tuning g_tuning{16, 16, 3000, 0};
+
+epcq_tuning current_tuning() {
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return g_tuning;
+}
+
+struct ViewArgs {
     const float* c2w;
     float fx, fy, cx, cy;
     int W, H;
-    int k, ko, L;  // window size, ceil(k/2), k*k
-    float r0, t_intvl, alpha_p, gamma;
-    int S, Kc;
-    const int* rays;  // nullptr = all pixels in order
-    int64_t R;
-    // dense drop-in layout
-    float* sp_xyz;
-    int* pc_idx;
-    float* sp_t;
-    uint8_t* mask;
-    // compact layout
-    int s_max;
-    int* n_samples;
-    float* sample_t;
-    int* nb_idx;
-    float* nb_t;
-    float* nb_dist;
-    const void* feat;
-    int feat_bf16, F;
-    float* w;
-    float* agg;
+    const int2* hash;
+    const int* sorted_idx;
 };
 
-__device__ __forceinline__ float radius_of(const SampleArgs& a, int id) {
-    return a.radius ? a.radius[id] : a.r0;
+__device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = v.c2w[i];
+    make_ray(m, v.fx, v.fy, v.cx, v.cy, u, vv, ray);
 }
 
-// Every point of the k*k (clamped) neighbour buckets whose point-to-ray
-// distance is below its radius, at most 48 per bucket, in lane order then
-// bucket order (quick_sampling_kernel :125-155).
-template <class Fn>
-__device__ __forceinline__ void for_each_ray_hit(const SampleArgs& a, const Ray& ray, int u, int v, Fn&& fn) {
-    for (int l = 0; l < a.L; ++l) {
-        const int h = neighbor_pixel(a.k, a.ko, l, u, v, a.W, a.H);
-        const int2 e = a.hash[h];
-        if (e.y <= 0) continue;
-        int kept = 0;
-        for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
-            const int id = a.sorted_idx[e.x + j];
-            const float* p = a.xyz + 3 * (int64_t)id;
-            float t, dist;
-            point_ray(ray, p[0], p[1], p[2], t, dist);
-            if (dist < radius_of(a, id)) {
-                ++kept;
-                fn(t, dist);
-            }
-        }
-    }
+template <int MODE, int KMAX, int SMAX>
+__global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
+                                                             int tiles_x) {
+    extern __shared__ float4 s_dyn[];
+    __shared__ int s_total;
+    float4* s_pts = s_dyn;
+    int2* s_hash = reinterpret_cast<int2*>(s_dyn + tp.cap);
+    int* s_rowlo = reinterpret_cast<int*>(s_hash + tp.rw_max * tp.rh_max);
+    int* s_rowbase = s_rowlo + tp.rh_max;
+
+    const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int u0 = (tile % tiles_x) * tp.tw;
+    const int v0 = (tile / tiles_x) * tp.th;
+    int ra, va, RW;
+    const int staged = stage_tile(view.hash, view.sorted_idx, o.xyz, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th,
+                                  tp.cap, s_pts, s_hash, s_rowlo, s_rowbase, &s_total, ra, va, RW);
+    if (staged < 0) return;  // region larger than the LDS capacity: sample_tile_fallback_kernel
+    const int u = u0 + (int)threadIdx.x % tp.tw;
+    const int v = v0 + (int)threadIdx.x / tp.tw;
+    if (u >= view.W || v >= view.H) return;
+    const int64_t ri = (int64_t)v * view.W + u;
+    Ray ray;
+    view_ray(view, u, v, ray);
+    const SrcLds src{s_hash, s_pts, ra, va, RW};
+    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, true);
 }
 
-__device__ __forceinline__ bool knn_less(float d, int id, float bd, int bi) {
-    return d < bd || (d == bd && id < bi);
+// The tiles sample_tile_kernel skipped (halo region above the LDS capacity),
+// sampled from global memory.
+template <int MODE, int KMAX, int SMAX>
+__global__ __launch_bounds__(kBlock) void sample_tile_fallback_kernel(SampleOpts o, SampleOut out, ViewArgs view,
+                                                                      TilePlan tp, int tiles_x) {
+    __shared__ int s_sum;
+    const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int u0 = (tile % tiles_x) * tp.tw;
+    const int v0 = (tile / tiles_x) * tp.th;
+    const int total = region_points(view.hash, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, &s_sum);
+    if (total <= tp.cap) return;
+    const int u = u0 + (int)threadIdx.x % tp.tw;
+    const int v = v0 + (int)threadIdx.x / tp.tw;
+    if (u >= view.W || v >= view.H) return;
+    const int64_t ri = (int64_t)v * view.W + u;
+    Ray ray;
+    view_ray(view, u, v, ray);
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.W};
+    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, true);
 }
 
-// Neighbours of one sample point; writes Kc indices (and, in the compact
-// layout, ray depths / distances / IDW outputs) for sample slot `so`.
-template <int MODE, int KMAX>
-__device__ void sample_neighbours(const SampleArgs& a, const Ray& ray, int u, int v, const float sp[3],
-                                  int64_t ri, int so, bool dense) {
-    float bd[KMAX];
-    int bi[KMAX];
-    int bsize = 0;
-    if (MODE == EPCQ_MODE_COMPAT) {
-        // quick_search_nearby_pc (:39-74): 8-entry buffer that tracks its
-        // minimum while filling, then replaces its tracked entry.
-        int maxi = 0;
-        float maxd = 1e3f;
-#pragma unroll
-        for (int b = 0; b < KMAX; ++b) {
-            bd[b] = 0.0f;
-            bi[b] = -1;
-        }
-        for (int l = 0; l < a.L; ++l) {
-            const int h = neighbor_pixel(a.k, a.ko, l, u, v, a.W, a.H);
-            const int2 e = a.hash[h];
-            if (e.y < 0) continue;
-            for (int j = 0; j < e.y; ++j) {
-                const int id = a.sorted_idx[e.x + j];
-                const float* p = a.xyz + 3 * (int64_t)id;
-                const float d = norm3(sp[0] - p[0], sp[1] - p[1], sp[2] - p[2]);
-                if (!(d < radius_of(a, id))) continue;
-                if (bsize < KMAX) {
-#pragma unroll
-                    for (int b = 0; b < KMAX; ++b)
-                        if (b == bsize) {
-                            bd[b] = d;
-                            bi[b] = id;
-                        }
-                    if (maxd > d) {
-                        maxd = d;
-                        maxi = bsize;
-                    }
-                    ++bsize;
-                } else if (d < maxd) {
-#pragma unroll
-                    for (int b = 0; b < KMAX; ++b)
-                        if (b == maxi) {
-                            bd[b] = d;
-                            bi[b] = id;
-                        }
-                    maxd = d;
-#pragma unroll
-                    for (int b = 0; b < KMAX; ++b) {
-                        if (bd[b] > maxd) {
-                            maxd = bd[b];
-                            maxi = b;
-                        }
-                    }
-                }
-            }
-        }
-    } else {
-        // The KMAX nearest by (distance, index) over the distinct window
-        // pixels, kept sorted; the first Kc <= KMAX of them are the Kc nearest.
-#pragma unroll
-        for (int b = 0; b < KMAX; ++b) {
-            bd[b] = __builtin_inff();
-            bi[b] = INT_MAX;
-        }
-        const int dv0 = 1 - a.ko, dv1 = a.k - a.ko;
-        const int du0 = -(a.ko - 1), du1 = a.k - a.ko;
-        const int va = max(0, v + dv0), vz = min(a.H - 1, v + dv1);
-        const int ua = max(0, u + du0), uz = min(a.W - 1, u + du1);
-        for (int vb = va; vb <= vz; ++vb) {
-            for (int ub = ua; ub <= uz; ++ub) {
-                const int2 e = a.hash[vb * a.W + ub];
-                for (int j = 0; j < e.y; ++j) {
-                    const int id = a.sorted_idx[e.x + j];
-                    const float* p = a.xyz + 3 * (int64_t)id;
-                    const float d = norm3(sp[0] - p[0], sp[1] - p[1], sp[2] - p[2]);
-                    if (!(d < radius_of(a, id))) continue;
-                    if (!knn_less(d, id, bd[KMAX - 1], bi[KMAX - 1])) continue;
-                    bool placed = false;
-#pragma unroll
-                    for (int b = KMAX - 1; b > 0; --b) {
-                        if (!placed) {
-                            if (knn_less(d, id, bd[b - 1], bi[b - 1])) {
-                                bd[b] = bd[b - 1];
-                                bi[b] = bi[b - 1];
-                            } else {
-                                bd[b] = d;
-                                bi[b] = id;
-                                placed = true;
-                            }
-                        }
-                    }
-                    if (!placed) {
-                        bd[0] = d;
-                        bi[0] = id;
-                    }
-                    if (bsize < KMAX) ++bsize;
-                }
-            }
-        }
-    }
-
-    const int nkeep = bsize < a.Kc ? bsize : a.Kc;
-    if (dense) {
-        int* out = a.pc_idx + (ri * a.S + so) * (int64_t)a.Kc;
-#pragma unroll
-        for (int b = 0; b < KMAX; ++b)
-            if (b < a.Kc) out[b] = b < nkeep ? bi[b] : -1;
-        for (int b = KMAX; b < a.Kc; ++b) out[b] = -1;
-        return;
-    }
-    const int64_t row = (ri * a.s_max + so) * (int64_t)a.Kc;
-    float den = 0.0f;
-    float inv[KMAX];
-#pragma unroll
-    for (int b = 0; b < KMAX; ++b) {
-        inv[b] = 0.0f;
-        if (b < a.Kc) {
-            const bool valid = b < nkeep;
-            a.nb_idx[row + b] = valid ? bi[b] : -1;
-            if (a.nb_t || a.nb_dist) {
-                float t = 0.0f, dist = 0.0f;
-                if (valid) {
-                    const float* p = a.xyz + 3 * (int64_t)bi[b];
-                    point_ray(ray, p[0], p[1], p[2], t, dist);
-                }
-                if (a.nb_t) a.nb_t[row + b] = t;
-                if (a.nb_dist) a.nb_dist[row + b] = dist;
-            }
-            if (valid) {
-                // (1 / (d + 1e-6)) * mask, epcq.py:436
-                inv[b] = 1.0f / (bd[b] + 1e-6f);
-                den = den + inv[b];
-            }
-        }
-    }
-    for (int b = KMAX; b < a.Kc; ++b) {
-        a.nb_idx[row + b] = -1;
-        if (a.nb_t) a.nb_t[row + b] = 0.0f;
-        if (a.nb_dist) a.nb_dist[row + b] = 0.0f;
-    }
-    if (!a.feat) return;
-    den = den + 1e-6f;
-    float* wo = a.w + row;
-#pragma unroll
-    for (int b = 0; b < KMAX; ++b) {
-        if (b < a.Kc) wo[b] = inv[b] / den;
-    }
-    for (int b = KMAX; b < a.Kc; ++b) wo[b] = 0.0f;
-    float* ao = a.agg + (ri * a.s_max + so) * (int64_t)a.F;
-    for (int f0 = 0; f0 < a.F; f0 += 4) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int b = 0; b < KMAX; ++b) {
-            if (b < nkeep) {
-                const float wb = inv[b] / den;
-                const int64_t base = (int64_t)bi[b] * a.F + f0;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    if (f0 + c < a.F) {
-                        float fv;
-                        if (a.feat_bf16) {
-                            const uint16_t hb = static_cast<const uint16_t*>(a.feat)[base + c];
-                            fv = __uint_as_float((uint32_t)hb << 16);
-                        } else {
-                            fv = static_cast<const float*>(a.feat)[base + c];
-                        }
-                        acc[c] = acc[c] + wb * fv;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (f0 + c < a.F) ao[f0 + c] = acc[c];
-    }
-}
-
-template <int MODE, int KMAX>
-__global__ __launch_bounds__(kBlock) void sample_kernel(SampleArgs a) {
+template <int MODE, int KMAX, int SMAX>
+__global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, SampleOut out, ViewArgs view,
+                                                             const int* __restrict__ rays, int64_t R) {
     const int64_t ri = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (ri >= a.R) return;
-    const bool dense = a.sp_xyz != nullptr;
-    const int64_t q = a.rays ? (int64_t)a.rays[ri] : ri;
-    const int64_t P = (int64_t)a.W * a.H;
-    int ns = 0;
-    if (q >= 0 && q < P) {
-        const int u = (int)(q % a.W), v = (int)(q / a.W);
-        float m[12];
-#pragma unroll
-        for (int i = 0; i < 12; ++i) m[i] = a.c2w[i];
-        Ray ray;
-        make_ray(m, a.fx, a.fy, a.cx, a.cy, u, v, ray);
-
-        // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
-        float tmin = 1e3f, tmax = -1e3f;
-        for_each_ray_hit(a, ray, u, v, [&](float t, float) {
-            if (tmin > t) tmin = t;
-            if (tmax < t) tmax = t;
-        });
-
-        if (!(tmin > tmax)) {
-            // Steps 2-3: slot sums and the front-to-back walk (:168-219).
-            float T = 1.0f;
-            int lo = 0;
-            bool done = false;
-            while (!done) {
-                int tslot[kSlotTable];
-                float tsum[kSlotTable], tcnt[kSlotTable];
-#pragma unroll
-                for (int e = 0; e < kSlotTable; ++e) {
-                    tslot[e] = INT_MAX;
-                    tsum[e] = 0.0f;
-                    tcnt[e] = 0.0f;
-                }
-                int nt = 0;
-                int limit = kMaxSlots;
-                for_each_ray_hit(a, ray, u, v, [&](float t, float dist) {
-                    const float qf = (t - tmin) / a.t_intvl;
-                    if (!(qf > -1.0f && qf < (float)kMaxSlots)) return;
-                    const int s = (int)qf;
-                    if (s < lo || s >= limit) return;
-                    bool found = false;
-#pragma unroll
-                    for (int e = 0; e < kSlotTable; ++e) {
-                        if (tslot[e] == s) {
-                            tsum[e] = tsum[e] + dist;
-                            tcnt[e] = tcnt[e] + 1.0f;
-                            found = true;
-                        }
-                    }
-                    if (found) return;
-                    if (nt < kSlotTable) {
-#pragma unroll
-                        for (int e = 0; e < kSlotTable; ++e)
-                            if (e == nt) {
-                                tslot[e] = s;
-                                tsum[e] = dist;
-                                tcnt[e] = 1.0f;
-                            }
-                        ++nt;
-                        return;
-                    }
-                    int mslot = -1, mx = 0;
-#pragma unroll
-                    for (int e = 0; e < kSlotTable; ++e)
-                        if (tslot[e] > mslot) {
-                            mslot = tslot[e];
-                            mx = e;
-                        }
-                    if (s > mslot) {
-                        limit = min(limit, s);
-                        return;
-                    }
-                    limit = mslot;  // the evicted slot and everything above wait for the next pass
-#pragma unroll
-                    for (int e = 0; e < kSlotTable; ++e)
-                        if (e == mx) {
-                            tslot[e] = s;
-                            tsum[e] = dist;
-                            tcnt[e] = 1.0f;
-                        }
-                });
-                // walk this window's slots in ascending order
-                int prev = lo - 1;
-                for (int it = 0; it < nt && !done; ++it) {
-                    int best = INT_MAX;
-                    float bsum = 0.0f, bcnt = 0.0f;
-#pragma unroll
-                    for (int e = 0; e < kSlotTable; ++e)
-                        if (tslot[e] > prev && tslot[e] < best) {
-                            best = tslot[e];
-                            bsum = tsum[e];
-                            bcnt = tcnt[e];
-                        }
-                    prev = best;
-                    if (bcnt < 2.0f) continue;
-                    if (T < 1e-3f || ns >= a.S) {
-                        done = true;
-                        break;
-                    }
-                    const float sdf = bsum / (bcnt + 1e-6f);
-                    const float alpha = a.gamma * expf_cw(-sdf / a.alpha_p);
-                    const float wgt = alpha * T;
-                    if (wgt > 1e-3f) {
-                        const float ts = tmin + (float)best * a.t_intvl;
-                        float sp[3];
-                        sp[0] = ray.o[0] + ts * ray.d[0];
-                        sp[1] = ray.o[1] + ts * ray.d[1];
-                        sp[2] = ray.o[2] + ts * ray.d[2];
-                        if (dense) {
-                            const int64_t o = ri * a.S + ns;
-                            a.sp_xyz[3 * o + 0] = sp[0];
-                            a.sp_xyz[3 * o + 1] = sp[1];
-                            a.sp_xyz[3 * o + 2] = sp[2];
-                            a.sp_t[o] = ts;
-                            a.mask[o] = 1;
-                            sample_neighbours<MODE, KMAX>(a, ray, u, v, sp, ri, ns, true);
-                        } else if (ns < a.s_max) {
-                            a.sample_t[ri * a.s_max + ns] = ts;
-                            sample_neighbours<MODE, KMAX>(a, ray, u, v, sp, ri, ns, false);
-                        }
-                        ++ns;
-                    }
-                    T -= alpha;
-                }
-                if (!done) {
-                    if (limit >= kMaxSlots) break;
-                    lo = limit;
-                }
-            }
-        }
-    }
-
-    // Everything not written above: zeros / -1 / false.
-    if (dense) {
-        for (int s = ns; s < a.S; ++s) {
-            const int64_t o = ri * a.S + s;
-            a.sp_xyz[3 * o + 0] = 0.0f;
-            a.sp_xyz[3 * o + 1] = 0.0f;
-            a.sp_xyz[3 * o + 2] = 0.0f;
-            a.sp_t[o] = 0.0f;
-            a.mask[o] = 0;
-            int* out = a.pc_idx + o * a.Kc;
-            for (int b = 0; b < a.Kc; ++b) out[b] = -1;
-        }
-    } else {
-        a.n_samples[ri] = ns;
-        for (int s = ns; s < a.s_max; ++s) {
-            const int64_t o = ri * a.s_max + s;
-            a.sample_t[o] = 0.0f;
-            for (int b = 0; b < a.Kc; ++b) {
-                a.nb_idx[o * a.Kc + b] = -1;
-                if (a.nb_t) a.nb_t[o * a.Kc + b] = 0.0f;
-                if (a.nb_dist) a.nb_dist[o * a.Kc + b] = 0.0f;
-                if (a.feat) a.w[o * a.Kc + b] = 0.0f;
-            }
-            if (a.feat)
-                for (int f = 0; f < a.F; ++f) a.agg[o * a.F + f] = 0.0f;
-        }
-    }
+    if (ri >= R) return;
+    const int64_t q = rays[ri];
+    const bool valid = q >= 0 && q < (int64_t)view.W * view.H;
+    const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
+    Ray ray;
+    view_ray(view, u, v, ray);
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.W};
+    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each
@@ -527,55 +236,107 @@ int check_common(const epcq_camera* cam, const epcq_options* opt, const char* wh
     if ((int64_t)cam->width * cam->height >= (int64_t(1) << 30))
         return fail(EPCQ_ERR_INVALID, std::string(who) + ": image too large");
     if (opt->kernel_size < 1) return fail(EPCQ_ERR_INVALID, std::string(who) + ": kernel_size must be >= 1");
+    if (opt->kernel_size > 64) return fail(EPCQ_ERR_INVALID, std::string(who) + ": kernel_size must be <= 64");
     if (!cam->c2w) return fail(EPCQ_ERR_INVALID, std::string(who) + ": null c2w");
     return EPCQ_OK;
 }
 
-SampleArgs make_args(const float* xyz, const epcq_camera* cam, const epcq_options* opt, const int* sorted_idx,
-                     const int* hash, const float* radius) {
-    SampleArgs a{};
-    a.xyz = xyz;
-    a.sorted_idx = sorted_idx;
-    a.hash = reinterpret_cast<const int2*>(hash);
-    a.radius = radius;
-    a.c2w = cam->c2w;
-    a.fx = cam->fx;
-    a.fy = cam->fy;
-    a.cx = cam->cx;
-    a.cy = cam->cy;
-    a.W = cam->width;
-    a.H = cam->height;
-    a.k = opt->kernel_size;
-    a.ko = (opt->kernel_size + 1) / 2;
-    a.L = opt->kernel_size * opt->kernel_size;
-    a.r0 = opt->radius_threshold;
-    a.t_intvl = opt->t_intvl;
-    a.alpha_p = opt->sdf2weight_gaussian_alpha;
-    a.gamma = opt->sdf2weight_gaussian_gamma;
-    a.S = opt->num_sample_points_per_ray;
-    a.Kc = opt->num_point_cloud_per_sp;
-    return a;
+SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* radius) {
+    SampleOpts o{};
+    o.k = opt->kernel_size;
+    o.ko = (opt->kernel_size + 1) / 2;
+    o.L = opt->kernel_size * opt->kernel_size;
+    o.r0 = opt->radius_threshold;
+    o.r2lim = squared_threshold(opt->radius_threshold);
+    o.radius = radius;
+    o.t_intvl = opt->t_intvl;
+    o.alpha_p = opt->sdf2weight_gaussian_alpha;
+    o.gamma = opt->sdf2weight_gaussian_gamma;
+    o.S = opt->num_sample_points_per_ray;
+    o.Kc = opt->num_point_cloud_per_sp;
+    o.xyz = xyz;
+    o.prefilter = (current_tuning().flags & EPCQ_TUNE_PREFILTER) != 0;
+    return o;
 }
 
-int launch_sample(const SampleArgs& a, int mode, hipStream_t st, const char* who) {
-    if (a.R == 0) return EPCQ_OK;
-    const dim3 grid((unsigned)ceil_div(a.R, kBlock)), block(kBlock);
-    if (mode == EPCQ_MODE_COMPAT) {
-        sample_kernel<EPCQ_MODE_COMPAT, kCompatBuffer><<<grid, block, 0, st>>>(a);
-    } else if (a.Kc <= 8) {
-        sample_kernel<EPCQ_MODE_KNN, 8><<<grid, block, 0, st>>>(a);
-    } else if (a.Kc <= 16) {
-        sample_kernel<EPCQ_MODE_KNN, 16><<<grid, block, 0, st>>>(a);
+ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash) {
+    return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
+                    reinterpret_cast<const int2*>(hash), sorted_idx};
+}
+
+template <int MODE, int KMAX, int SMAX>
+void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                const TilePlan& tp, hipStream_t st) {
+    if (rays) {
+        sample_list_kernel<MODE, KMAX, SMAX><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
+            o, out, view, rays, R);
     } else {
-        sample_kernel<EPCQ_MODE_KNN, 32><<<grid, block, 0, st>>>(a);
+        const int tiles_x = (int)ceil_div(view.W, tp.tw);
+        const int tiles = tiles_x * (int)ceil_div(view.H, tp.th);
+        const dim3 block(tp.tw * tp.th);
+        sample_tile_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp, tiles_x);
+        sample_tile_fallback_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, 0, st>>>(o, out, view, tp, tiles_x);
+    }
+}
+
+template <int MODE, int KMAX>
+void launch_smax(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                 const TilePlan& tp, int bound, hipStream_t st) {
+    if (bound <= 2) launch_one<MODE, KMAX, 2>(o, out, view, rays, R, tp, st);
+    else launch_one<MODE, KMAX, kMaxLiveSamples>(o, out, view, rays, R, tp, st);
+}
+
+// `bound`: samples one ray can emit (<= kMaxLiveSamples, checked by callers).
+int launch_sample(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                  int mode, int bound, hipStream_t st, const char* who) {
+    if (R == 0) return EPCQ_OK;
+    const epcq_tuning tu = current_tuning();
+    const TilePlan tp = plan_tiles(o.k, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags);
+    if (mode == EPCQ_MODE_COMPAT) {
+        launch_smax<EPCQ_MODE_COMPAT, kCompatBuffer>(o, out, view, rays, R, tp, bound, st);
+    } else if (o.Kc <= 8) {
+        launch_smax<EPCQ_MODE_KNN, 8>(o, out, view, rays, R, tp, bound, st);
+    } else if (o.Kc <= 16) {
+        launch_smax<EPCQ_MODE_KNN, 16>(o, out, view, rays, R, tp, bound, st);
+    } else {
+        launch_smax<EPCQ_MODE_KNN, 32>(o, out, view, rays, R, tp, bound, st);
     }
     return check_launch(who);
+}
+
+int live_bound(const epcq_options* opt, float r_max, const char* who, int* bound) {
+    *bound = epcq_max_live_samples(opt, r_max);
+    if (*bound > kMaxLiveSamples)
+        return fail(EPCQ_ERR_INVALID, std::string(who) + ": these options allow more than " +
+                                          std::to_string(kMaxLiveSamples) + " samples per ray");
+    return EPCQ_OK;
 }
 
 }  // namespace
 }  // namespace epcq
 
 using namespace epcq;
+
+extern "C" int epcq_set_tuning(const epcq_tuning* t) {
+    if (!t) return fail(EPCQ_ERR_INVALID, "set_tuning: null");
+    const int rays = t->tile_w * t->tile_h;
+    if (t->tile_w < 1 || t->tile_w > 64 || (t->tile_w & (t->tile_w - 1)) || t->tile_h < 1 ||
+        (rays != 64 && rays != 128 && rays != 256))
+        return fail(EPCQ_ERR_INVALID,
+                    "set_tuning: tile_w must be a power of two <= 64 and tile_w * tile_h 64, 128 or 256");
+    if (t->tile_cap < 0) return fail(EPCQ_ERR_INVALID, "set_tuning: negative tile_cap");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning = *t;
+    return EPCQ_OK;
+}
+
+extern "C" int epcq_get_tuning(epcq_tuning* t) {
+    if (!t) return fail(EPCQ_ERR_INVALID, "get_tuning: null");
+    *t = current_tuning();
+    return EPCQ_OK;
+}
+
+extern "C" float epcq_squared_threshold(float r) { return squared_threshold(r); }
 
 extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max) {
     // Every visited slot (count >= 2) lowers the transmittance by
@@ -609,14 +370,17 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
     if (mode == EPCQ_MODE_KNN && opt->num_point_cloud_per_sp > 32)
         return fail(EPCQ_ERR_INVALID, "quick_sampling: num_point_cloud_per_sp > 32 in KNN mode");
     (void)n;
-    SampleArgs a = make_args(xyz, cam, opt, sorted_idx, hash, nullptr);
-    a.R = (int64_t)cam->width * cam->height;
-    a.sp_xyz = sp_xyz;
-    a.pc_idx = pc_idx;
-    a.sp_t = sp_t;
-    a.mask = mask;
-    if (a.S == 0) return EPCQ_OK;
-    return launch_sample(a, mode, as_stream(stream), "quick_sampling");
+    if (opt->num_sample_points_per_ray == 0) return EPCQ_OK;
+    int bound;
+    if (int e = live_bound(opt, opt->radius_threshold, "quick_sampling", &bound)) return e;
+    const SampleOpts o = make_opts(xyz, opt, nullptr);
+    SampleOut out{};
+    out.sp_xyz = sp_xyz;
+    out.pc_idx = pc_idx;
+    out.sp_t = sp_t;
+    out.mask = mask;
+    return launch_sample(o, out, make_view(cam, sorted_idx, hash), nullptr, (int64_t)cam->width * cam->height, mode,
+                         bound, as_stream(stream), "quick_sampling");
 }
 
 extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
@@ -632,23 +396,32 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     if (s_max < 1) return fail(EPCQ_ERR_INVALID, "knn_sampling: s_max must be >= 1");
     if (!n_samples || !sample_t || !nb_idx) return fail(EPCQ_ERR_INVALID, "knn_sampling: null output");
     if (feat && (F < 1 || !w || !agg)) return fail(EPCQ_ERR_INVALID, "knn_sampling: bad feature arguments");
+    if (rays && R < 0) return fail(EPCQ_ERR_INVALID, "knn_sampling: negative ray count");
     (void)n;
-    SampleArgs a = make_args(xyz, cam, opt, sorted_idx, hash, radius);
-    a.rays = rays;
-    a.R = rays ? R : (int64_t)cam->width * cam->height;
-    if (a.R < 0) return fail(EPCQ_ERR_INVALID, "knn_sampling: negative ray count");
-    a.s_max = s_max;
-    a.n_samples = n_samples;
-    a.sample_t = sample_t;
-    a.nb_idx = nb_idx;
-    a.nb_t = nb_t;
-    a.nb_dist = nb_dist;
-    a.feat = feat;
-    a.feat_bf16 = feat_bf16;
-    a.F = feat ? F : 0;
-    a.w = w;
-    a.agg = agg;
-    return launch_sample(a, mode, as_stream(stream), "knn_sampling");
+    // with per-point radii the caller's s_max stands for the bound
+    int bound = s_max;
+    if (!radius) {
+        if (int e = live_bound(opt, opt->radius_threshold, "knn_sampling", &bound)) return e;
+        bound = std::min(bound, s_max);
+    } else if (bound > kMaxLiveSamples) {
+        return fail(EPCQ_ERR_INVALID, "knn_sampling: s_max above 16 with per-point radius");
+    }
+    SampleOpts o = make_opts(xyz, opt, radius);
+    o.feat = feat;
+    o.feat_bf16 = feat_bf16;
+    o.F = feat ? F : 0;
+    SampleOut out{};
+    out.s_max = s_max;
+    out.n_samples = n_samples;
+    out.sample_t = sample_t;
+    out.nb_idx = nb_idx;
+    out.nb_t = nb_t;
+    out.nb_dist = nb_dist;
+    out.w = w;
+    out.agg = agg;
+    const int64_t nr = rays ? R : (int64_t)cam->width * cam->height;
+    return launch_sample(o, out, make_view(cam, sorted_idx, hash), rays, nr, mode, bound, as_stream(stream),
+                         "knn_sampling");
 }
 
 extern "C" int epcq_quick_query(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,

@@ -1,0 +1,739 @@
+// Per-ray sampling core shared by the per-view and batched sampling kernels.
+//
+// One lane owns one ray.  run_ray() performs, for the ray through pixel
+// (u, v):
+//   1. the hit walk of quick_sampling_kernel (epcq/csrc/epcq_render_topk.cu
+//      :125-161): every point of the k*k clamped neighbour buckets whose
+//      point-to-ray distance is below its radius, at most 48 per bucket, in
+//      lane order then bucket order, giving tmin / tmax;
+//   2. the 0.01-wide depth-slot histogram and the front-to-back walk that
+//      emits up to S samples (:168-219);
+//   3. for each emitted sample, its neighbours: the reference's 8-entry
+//      buffer (quick_search_nearby_pc, :39-79) or the K nearest by
+//      (distance, index), with optional inverse-distance weighted features
+//      (epcq/epcq.py:434-437,449-452).
+// Hits are never stored: the candidate walk is repeated for each step, so all
+// per-lane state stays in registers.  The walks are: tmin / tmax; slot
+// occupancy (once / at least twice) with the distance sums of the lowest
+// occupied slots; a refill walk only when the front-to-back walk reaches a
+// slot the table does not hold; one neighbour walk per two sample points.
+// Sums are accumulated in the canonical (lane, bucket position) order, so the
+// result equals a full 128-slot histogram.  A conservative cross-product test
+// rejects candidates that are certainly outside the radius before the exact
+// projection.
+//
+// Candidates come from a source `Src` with
+//   int2   bucket(ub, vb)  (start, count) of pixel (ub, vb), count -1 if empty
+//   float4 point(j)        (x, y, z, index bits) of list position j
+// either global memory (SrcGlobal) or an LDS copy of the tile's halo region
+// (SrcLds); each kernel instantiates exactly one, so LDS reads stay ds_read
+// instructions.
+//
+// With a uniform radius r the hit test sqrtf(e2) < r is evaluated exactly as
+// e2 < r2lim, where r2lim is the smallest float whose correctly rounded
+// square root is >= r (sqrt is monotone), so square roots are only taken for
+// hits.
+#pragma once
+
+#include <climits>
+
+#include "epcq_device.h"
+
+namespace epcq {
+
+constexpr int kSlotTable = 8;
+
+struct SampleOpts {
+    int k, ko, L;         // window size, ceil(k/2), k*k
+    float r0, r2lim;      // uniform radius and its exact squared threshold
+    const float* radius;  // per-point radius or nullptr
+    float t_intvl, alpha_p, gamma;
+    int S, Kc;
+    const float* xyz;
+    const void* feat;
+    int feat_bf16, F;
+    int prefilter;        // cross-product rejection before the exact test
+};
+
+struct SampleOut {
+    // dense drop-in layout [R,S,...] (sp_xyz != nullptr)
+    float* sp_xyz;
+    int* pc_idx;
+    float* sp_t;
+    uint8_t* mask;
+    // compact layout [R,s_max,...]
+    int s_max;
+    int* n_samples;
+    float* sample_t;
+    int* nb_idx;
+    float* nb_t;
+    float* nb_dist;
+    float* w;
+    float* agg;
+};
+
+struct SrcGlobal {
+    const int2* hash;
+    const int* sorted_idx;
+    const float* xyz;
+    int W;
+    __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[vb * W + ub]; }
+    __device__ __forceinline__ float4 point(int j) const {
+        const int id = sorted_idx[j];
+        const float* p = xyz + 3 * (int64_t)id;
+        return make_float4(p[0], p[1], p[2], __int_as_float(id));
+    }
+};
+
+struct SrcLds {
+    const int2* hash;
+    const float4* pts;
+    int ra, va, RW;
+    __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float4 point(int j) const { return pts[j]; }
+};
+
+// PointRayInterset up to the squared distance: t and |p - (o + t d)|^2.
+__device__ __forceinline__ void ray_t_e2(const Ray& r, const float4& p, float& t, float& e2) {
+    const float w0 = p.x - r.o[0], w1 = p.y - r.o[1], w2 = p.z - r.o[2];
+    t = dot3(w0, w1, w2, r.d[0], r.d[1], r.d[2]) / r.dd;
+    const float q0 = r.o[0] + t * r.d[0];
+    const float q1 = r.o[1] + t * r.d[1];
+    const float q2 = r.o[2] + t * r.d[2];
+    e2 = sqnorm3(p.x - q0, p.y - q1, p.z - q2);
+}
+
+// Conservative rejection before the exact test: true only if the point is
+// certainly at distance >= r from the ray.  |w x d| / |d| is the exact
+// distance of o + w from the ray; its fused-multiply-add estimate differs by
+// at most ~4 eps |w|, and the canonical rounded distance of ray_t_e2 by at
+// most ~eps (18 |w| + 4 |o| + 1) (rounding of t, the foot point and the norm).
+// The margin 3e-6 (1 + |w|_1 + |o|_1) exceeds their sum by more than 2x, so a
+// point rejected here fails the exact test too.
+__device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, float w2, float r, float omargin) {
+    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
+    const float rm = __builtin_fmaf(3e-6f, aw, r + omargin);
+    const float c0 = __builtin_fmaf(w1, ray.d[2], -(w2 * ray.d[1]));
+    const float c1 = __builtin_fmaf(w2, ray.d[0], -(w0 * ray.d[2]));
+    const float c2 = __builtin_fmaf(w0, ray.d[1], -(w1 * ray.d[0]));
+    const float cs = __builtin_fmaf(c2, c2, __builtin_fmaf(c1, c1, c0 * c0));
+    return cs > rm * rm * ray.dd;
+}
+
+// dist < radius for a squared distance, exact (see the file comment).
+__device__ __forceinline__ bool within(const SampleOpts& o, float e2, int id) {
+    return o.radius ? sqrtf(e2) < o.radius[id] : e2 < o.r2lim;
+}
+
+// Calls fn(ub, vb) for the k*k neighbour lanes in order: lane l is
+// (du, dv) = (l % k - (ko - 1), (l + k) / k - ko), i.e. rows dv = 1 - ko ..
+// k - ko, columns du = 1 - ko .. k - ko, clamped to the image
+// (calculate_neighbor_uv, cuda_util.cuh:382-388).
+template <class Fn>
+__device__ __forceinline__ void for_each_lane(const SampleOpts& o, int u, int v, int W, int H, Fn&& fn) {
+    for (int dv = 1 - o.ko; dv <= o.k - o.ko; ++dv) {
+        const int vb = min(max(v + dv, 0), H - 1);
+        for (int du = 1 - o.ko; du <= o.k - o.ko; ++du) {
+            const int ub = min(max(u + du, 0), W - 1);
+            fn(ub, vb);
+        }
+    }
+}
+
+// Calls fn(t, e2) for each hit of the ray: the points of its neighbour
+// buckets within their radius, at most 48 per bucket, in lane then bucket
+// order (quick_sampling_kernel :125-155).  The next point is read before the
+// current one is tested.
+template <class Src, class Fn>
+__device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
+                                                 int W, int H, Fn&& fn) {
+    for_each_lane(o, u, v, W, H, [&](int ub, int vb) {
+        const int2 e = src.bucket(ub, vb);
+        if (e.y <= 0) return;
+        int kept = 0;
+        float4 nxt = src.point(e.x);
+        for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
+            const float4 p = nxt;
+            if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
+            const int id = __float_as_int(p.w);
+            const float r = o.radius ? o.radius[id] : o.r0;
+            if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))
+                continue;
+            float t, e2;
+            ray_t_e2(ray, p, t, e2);
+            if (within(o, e2, id)) {
+                ++kept;
+                fn(t, e2);
+            }
+        }
+    });
+}
+
+// 128-bit set of depth slots, one named word per 32 slots (kept in registers).
+struct SlotMask {
+    uint32_t w0, w1, w2, w3;
+};
+
+// Bit `s` as the four words of a SlotMask (0 <= s < 128), without branches.
+__device__ __forceinline__ SlotMask slot_bit(int s) {
+    const uint32_t bit = 1u << (s & 31);
+    const int w = s >> 5;
+    return SlotMask{w == 0 ? bit : 0u, w == 1 ? bit : 0u, w == 2 ? bit : 0u, w == 3 ? bit : 0u};
+}
+
+__device__ __forceinline__ uint32_t mask_from(uint32_t word, int lo) {
+    return lo >= 32 ? 0u : (lo > 0 ? word & ~((1u << lo) - 1u) : word);
+}
+
+// First slot >= from in m, or kMaxSlots.
+__device__ __forceinline__ int next_slot(const SlotMask& m, int from) {
+    const uint32_t a = mask_from(m.w0, from), b = mask_from(m.w1, from - 32);
+    const uint32_t c = mask_from(m.w2, from - 64), d = mask_from(m.w3, from - 96);
+    if (a) return __ffs(a) - 1;
+    if (b) return 32 + __ffs(b) - 1;
+    if (c) return 64 + __ffs(c) - 1;
+    if (d) return 96 + __ffs(d) - 1;
+    return kMaxSlots;
+}
+
+__device__ __forceinline__ bool knn_less(float d, int id, float bd, int bi) {
+    return d < bd || (d == bd && id < bi);
+}
+
+// Neighbour buffer of one sample point.
+template <int MODE, int KMAX>
+struct NbBuf {
+    float bd[KMAX];
+    int bi[KMAX];
+    int bsize;
+    int maxi;
+    float maxd;
+
+    __device__ __forceinline__ void init() {
+        bsize = 0;
+        maxi = 0;
+        maxd = 1e3f;
+#pragma unroll
+        for (int b = 0; b < KMAX; ++b) {
+            bd[b] = MODE == EPCQ_MODE_COMPAT ? 0.0f : __builtin_inff();
+            bi[b] = MODE == EPCQ_MODE_COMPAT ? -1 : INT_MAX;
+        }
+    }
+
+    __device__ __forceinline__ void add(float d, int id) {
+        if (MODE == EPCQ_MODE_COMPAT) {
+            // quick_search_nearby_pc (:50-73): while filling it tracks the
+            // minimum; once full a closer point replaces the tracked entry
+            // and the maximum is tracked from then on.
+            if (bsize < KMAX) {
+#pragma unroll
+                for (int b = 0; b < KMAX; ++b)
+                    if (b == bsize) {
+                        bd[b] = d;
+                        bi[b] = id;
+                    }
+                if (maxd > d) {
+                    maxd = d;
+                    maxi = bsize;
+                }
+                ++bsize;
+            } else if (d < maxd) {
+#pragma unroll
+                for (int b = 0; b < KMAX; ++b)
+                    if (b == maxi) {
+                        bd[b] = d;
+                        bi[b] = id;
+                    }
+                maxd = d;
+#pragma unroll
+                for (int b = 0; b < KMAX; ++b) {
+                    if (bd[b] > maxd) {
+                        maxd = bd[b];
+                        maxi = b;
+                    }
+                }
+            }
+        } else {
+            // sorted insertion: the KMAX nearest by (distance, index)
+            if (!knn_less(d, id, bd[KMAX - 1], bi[KMAX - 1])) return;
+            bool placed = false;
+#pragma unroll
+            for (int b = KMAX - 1; b > 0; --b) {
+                if (!placed) {
+                    if (knn_less(d, id, bd[b - 1], bi[b - 1])) {
+                        bd[b] = bd[b - 1];
+                        bi[b] = bi[b - 1];
+                    } else {
+                        bd[b] = d;
+                        bi[b] = id;
+                        placed = true;
+                    }
+                }
+            }
+            if (!placed) {
+                bd[0] = d;
+                bi[0] = id;
+            }
+            if (bsize < KMAX) ++bsize;
+        }
+    }
+};
+
+// One walk over the candidates of the window feeding the buffers of the
+// first `ns` (<= NS) sample points sp[].  Compat mode walks the k*k lanes
+// with their border duplicates, like quick_search_nearby_pc (:39-47); KNN
+// mode walks the distinct window pixels.
+template <int MODE, int KMAX, int NS, class Src>
+__device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
+                                               const float (&sp)[NS][3], int ns, NbBuf<MODE, KMAX> (&buf)[NS]) {
+    auto visit = [&](const float4& p) {
+        const int id = __float_as_int(p.w);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s < ns) {
+                const float d2 = sqnorm3(sp[s][0] - p.x, sp[s][1] - p.y, sp[s][2] - p.z);
+                if (within(o, d2, id)) buf[s].add(sqrtf(d2), id);
+            }
+        }
+    };
+    auto bucket = [&](int ub, int vb) {
+        const int2 e = src.bucket(ub, vb);
+        if (e.y <= 0) return;
+        float4 nxt = src.point(e.x);
+        for (int j = 0; j < e.y; ++j) {
+            const float4 p = nxt;
+            if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
+            visit(p);
+        }
+    };
+    if (MODE == EPCQ_MODE_COMPAT) {
+        for_each_lane(o, u, v, W, H, bucket);
+    } else {
+        const int ua = max(0, u - (o.ko - 1)), uz = min(W - 1, u + o.k - o.ko);
+        const int va = max(0, v + 1 - o.ko), vz = min(H - 1, v + o.k - o.ko);
+        for (int vb = va; vb <= vz; ++vb)
+            for (int ub = ua; ub <= uz; ++ub) bucket(ub, vb);
+    }
+}
+
+// Writes the neighbour outputs of sample slot `so` of ray `ri`.
+template <int MODE, int KMAX>
+__device__ __forceinline__ void write_neighbours(const SampleOpts& o, const SampleOut& out, const Ray& ray,
+                                                 const NbBuf<MODE, KMAX>& nb, int64_t ri, int so) {
+    const int nkeep = nb.bsize < o.Kc ? nb.bsize : o.Kc;
+    if (out.sp_xyz) {
+        int* dst = out.pc_idx + (ri * o.S + so) * (int64_t)o.Kc;
+#pragma unroll
+        for (int b = 0; b < KMAX; ++b)
+            if (b < o.Kc) dst[b] = b < nkeep ? nb.bi[b] : -1;
+        for (int b = KMAX; b < o.Kc; ++b) dst[b] = -1;
+        return;
+    }
+    const int64_t row = (ri * out.s_max + so) * (int64_t)o.Kc;
+    float den = 0.0f;
+    float inv[KMAX];
+#pragma unroll
+    for (int b = 0; b < KMAX; ++b) {
+        inv[b] = 0.0f;
+        if (b < o.Kc) {
+            const bool valid = b < nkeep;
+            out.nb_idx[row + b] = valid ? nb.bi[b] : -1;
+            if (out.nb_t || out.nb_dist) {
+                float t = 0.0f, dist = 0.0f;
+                if (valid) {
+                    const float* p = o.xyz + 3 * (int64_t)nb.bi[b];
+                    float e2;
+                    ray_t_e2(ray, make_float4(p[0], p[1], p[2], 0.0f), t, e2);
+                    dist = sqrtf(e2);
+                }
+                if (out.nb_t) out.nb_t[row + b] = t;
+                if (out.nb_dist) out.nb_dist[row + b] = dist;
+            }
+            if (valid) {
+                inv[b] = 1.0f / (nb.bd[b] + 1e-6f);  // (1 / (d + 1e-6)) * mask, epcq.py:436
+                den = den + inv[b];
+            }
+        }
+    }
+    for (int b = KMAX; b < o.Kc; ++b) {
+        out.nb_idx[row + b] = -1;
+        if (out.nb_t) out.nb_t[row + b] = 0.0f;
+        if (out.nb_dist) out.nb_dist[row + b] = 0.0f;
+    }
+    if (!o.feat) return;
+    den = den + 1e-6f;
+    float* wo = out.w + row;
+#pragma unroll
+    for (int b = 0; b < KMAX; ++b)
+        if (b < o.Kc) wo[b] = inv[b] / den;
+    for (int b = KMAX; b < o.Kc; ++b) wo[b] = 0.0f;
+    float* ao = out.agg + (ri * out.s_max + so) * (int64_t)o.F;
+    for (int f0 = 0; f0 < o.F; f0 += 4) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < KMAX; ++b) {
+            if (b < nkeep) {
+                const float wb = inv[b] / den;
+                const int64_t base = (int64_t)nb.bi[b] * o.F + f0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (f0 + c < o.F) {
+                        float fv;
+                        if (o.feat_bf16) {
+                            const uint16_t hb = static_cast<const uint16_t*>(o.feat)[base + c];
+                            fv = __uint_as_float((uint32_t)hb << 16);
+                        } else {
+                            fv = static_cast<const float*>(o.feat)[base + c];
+                        }
+                        acc[c] = acc[c] + wb * fv;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (f0 + c < o.F) ao[f0 + c] = acc[c];
+    }
+}
+
+// Samples and neighbours of one ray; writes every output element of row ri.
+// SMAX bounds the samples one ray can emit (epcq_max_live_samples).
+template <int MODE, int KMAX, int SMAX, class Src>
+__device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
+                        int W, int H, int64_t ri, bool valid_ray) {
+    constexpr int NS = KMAX <= 8 ? 2 : 1;  // sample points served by one neighbour walk
+    const bool dense = out.sp_xyz != nullptr;
+    int ns = 0;
+    float ts_arr[SMAX];
+#pragma unroll
+    for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
+    if (valid_ray) {
+        // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
+        float tmin = 1e3f, tmax = -1e3f;
+        for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float) {
+            if (tmin > t) tmin = t;
+            if (tmax < t) tmax = t;
+        });
+        if (!(tmin > tmax)) {
+            // Steps 2-3: slot sums and the front-to-back walk (:168-219).
+            // One pass marks the slots hit once and at least twice (only the
+            // latter take part in the walk) and sums the lowest occupied
+            // slots in a table; when the walk reaches a slot the table does
+            // not hold, a refill pass sums exactly the next table-full of
+            // slots with at least two hits.
+            SlotMask seen1{0u, 0u, 0u, 0u}, seen2{0u, 0u, 0u, 0u};
+            int tslot[kSlotTable];
+            float tsum[kSlotTable], tcnt[kSlotTable];
+#pragma unroll
+            for (int e = 0; e < kSlotTable; ++e) {
+                tslot[e] = INT_MAX;
+                tsum[e] = 0.0f;
+                tcnt[e] = 0.0f;
+            }
+            int nt = 0;
+            int limit = kMaxSlots;  // the table holds complete sums of every occupied slot below this
+            for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float e2) {
+                const float qf = (t - tmin) / o.t_intvl;
+                if (!(qf > -1.0f && qf < (float)kMaxSlots)) return;
+                const int s = (int)qf;
+                const SlotMask b = slot_bit(s);
+                seen2.w0 |= seen1.w0 & b.w0;
+                seen2.w1 |= seen1.w1 & b.w1;
+                seen2.w2 |= seen1.w2 & b.w2;
+                seen2.w3 |= seen1.w3 & b.w3;
+                seen1.w0 |= b.w0;
+                seen1.w1 |= b.w1;
+                seen1.w2 |= b.w2;
+                seen1.w3 |= b.w3;
+                if (s >= limit) return;
+                const float dist = sqrtf(e2);
+                bool found = false;
+#pragma unroll
+                for (int e = 0; e < kSlotTable; ++e) {
+                    if (tslot[e] == s) {
+                        tsum[e] = tsum[e] + dist;
+                        tcnt[e] = tcnt[e] + 1.0f;
+                        found = true;
+                    }
+                }
+                if (found) return;
+                if (nt < kSlotTable) {
+#pragma unroll
+                    for (int e = 0; e < kSlotTable; ++e)
+                        if (e == nt) {
+                            tslot[e] = s;
+                            tsum[e] = dist;
+                            tcnt[e] = 1.0f;
+                        }
+                    ++nt;
+                    return;
+                }
+                int mslot = -1, mx = 0;
+#pragma unroll
+                for (int e = 0; e < kSlotTable; ++e)
+                    if (tslot[e] > mslot) {
+                        mslot = tslot[e];
+                        mx = e;
+                    }
+                if (s > mslot) {
+                    limit = min(limit, s);
+                    return;
+                }
+                limit = mslot;  // the evicted slot and everything above leave the table
+#pragma unroll
+                for (int e = 0; e < kSlotTable; ++e)
+                    if (e == mx) {
+                        tslot[e] = s;
+                        tsum[e] = dist;
+                        tcnt[e] = 1.0f;
+                    }
+            });
+            float T = 1.0f;
+            int prev = -1;
+            while (true) {
+                const int s = next_slot(seen2, prev + 1);
+                if (s >= kMaxSlots) break;
+                prev = s;
+                if (s >= limit) {
+                    // refill: the next kSlotTable slots with >= 2 hits, from s
+                    int from = s;
+#pragma unroll
+                    for (int e = 0; e < kSlotTable; ++e) {
+                        tslot[e] = from < kMaxSlots ? from : INT_MAX;
+                        tsum[e] = 0.0f;
+                        tcnt[e] = 0.0f;
+                        from = from < kMaxSlots ? next_slot(seen2, from + 1) : kMaxSlots;
+                    }
+                    limit = from;
+                    const int first = s;
+                    for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float e2) {
+                        const float qf = (t - tmin) / o.t_intvl;
+                        if (!(qf > -1.0f && qf < (float)kMaxSlots)) return;
+                        const int q = (int)qf;
+                        if (q < first || q >= limit) return;
+                        const float dist = sqrtf(e2);
+#pragma unroll
+                        for (int e = 0; e < kSlotTable; ++e) {
+                            if (tslot[e] == q) {
+                                tsum[e] = tsum[e] + dist;
+                                tcnt[e] = tcnt[e] + 1.0f;
+                            }
+                        }
+                    });
+                }
+                float bsum = 0.0f, bcnt = 0.0f;
+#pragma unroll
+                for (int e = 0; e < kSlotTable; ++e)
+                    if (tslot[e] == s) {
+                        bsum = tsum[e];
+                        bcnt = tcnt[e];
+                    }
+                if (T < 1e-3f || ns >= o.S) break;
+                const float sdf = bsum / (bcnt + 1e-6f);
+                const float alpha = o.gamma * expf_cw(-sdf / o.alpha_p);
+                const float wgt = alpha * T;
+                if (wgt > 1e-3f) {
+                    const float ts = tmin + (float)s * o.t_intvl;
+                    const float s0 = ray.o[0] + ts * ray.d[0];
+                    const float s1 = ray.o[1] + ts * ray.d[1];
+                    const float s2 = ray.o[2] + ts * ray.d[2];
+                    if (dense) {
+                        const int64_t at = ri * o.S + ns;
+                        out.sp_xyz[3 * at + 0] = s0;
+                        out.sp_xyz[3 * at + 1] = s1;
+                        out.sp_xyz[3 * at + 2] = s2;
+                        out.sp_t[at] = ts;
+                        out.mask[at] = 1;
+                    } else if (ns < out.s_max) {
+                        out.sample_t[ri * out.s_max + ns] = ts;
+                    }
+#pragma unroll
+                    for (int q = 0; q < SMAX; ++q)
+                        if (q == ns) ts_arr[q] = ts;
+                    ++ns;
+                }
+                T -= alpha;
+            }
+        }
+        // Step 4: neighbours of the emitted samples, NS per walk.
+        const int nstore = min(min(ns, SMAX), dense ? o.S : out.s_max);
+        for (int g = 0; g < nstore; g += NS) {
+            float pend[NS][3];
+            NbBuf<MODE, KMAX> buf[NS];
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                float tq = 0.0f;
+#pragma unroll
+                for (int e = 0; e < SMAX; ++e)
+                    if (e == g + q) tq = ts_arr[e];
+                pend[q][0] = ray.o[0] + tq * ray.d[0];
+                pend[q][1] = ray.o[1] + tq * ray.d[1];
+                pend[q][2] = ray.o[2] + tq * ray.d[2];
+                buf[q].init();
+            }
+            const int cnt = min(NS, nstore - g);
+            neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, buf);
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+                if (q < cnt) write_neighbours<MODE, KMAX>(o, out, ray, buf[q], ri, g + q);
+        }
+    }
+
+    // Everything not written above: zeros / -1 / false.
+    if (dense) {
+        for (int s = ns; s < o.S; ++s) {
+            const int64_t at = ri * o.S + s;
+            out.sp_xyz[3 * at + 0] = 0.0f;
+            out.sp_xyz[3 * at + 1] = 0.0f;
+            out.sp_xyz[3 * at + 2] = 0.0f;
+            out.sp_t[at] = 0.0f;
+            out.mask[at] = 0;
+            int* dst = out.pc_idx + at * o.Kc;
+            for (int b = 0; b < o.Kc; ++b) dst[b] = -1;
+        }
+    } else {
+        out.n_samples[ri] = ns;
+        for (int s = ns; s < out.s_max; ++s) {
+            const int64_t at = ri * out.s_max + s;
+            out.sample_t[at] = 0.0f;
+            for (int b = 0; b < o.Kc; ++b) {
+                out.nb_idx[at * o.Kc + b] = -1;
+                if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
+                if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
+                if (o.feat) out.w[at * o.Kc + b] = 0.0f;
+            }
+            if (o.feat)
+                for (int f = 0; f < o.F; ++f) out.agg[at * o.F + f] = 0.0f;
+        }
+    }
+}
+
+// Smallest float s with sqrtf(s) >= r, so that sqrtf(e2) < r <=> e2 < s.
+float squared_threshold(float r);
+
+// Maps block b of nb to a tile index so that the blocks the hardware places
+// on one XCD (b, b + 8, b + 16, ...) take consecutive tiles and share halos
+// in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int per = nb / 8, rem = nb % 8;
+    const int x = b % 8, j = b / 8;
+    return x * per + min(x, rem) + j;
+}
+
+// Tile geometry of the LDS-staged sampler: a tw x th block of rays stages the
+// hash entries and points of its halo region (tw + k - 1) x (th + k - 1).
+struct TilePlan {
+    int tw, th;      // tile size in rays (tw * th == block size)
+    int xcd;         // XCD-contiguous tile order
+    int cap;         // staged points per tile
+    int rw_max, rh_max;
+    size_t lds_bytes;
+};
+
+TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags);
+
+// Halo region of the tile with top-left ray (u0, v0): pixels [ra, ra+RW) x
+// [va, va+RH) of the image.
+__device__ __forceinline__ void tile_region(int W, int H, int k, int ko, int u0, int v0, int tw, int th, int& ra,
+                                            int& va, int& RW, int& RH) {
+    ra = max(0, u0 - (ko - 1));
+    const int rz = min(W - 1, u0 + tw - 1 + (k - ko));
+    va = max(0, v0 + 1 - ko);
+    const int vz = min(H - 1, v0 + th - 1 + (k - ko));
+    RW = rz - ra + 1;
+    RH = vz - va + 1;
+}
+
+// Number of points stage_tile would copy for this tile (block-wide, reads
+// the hash entries from global memory; every thread gets the result).
+__device__ __forceinline__ int region_points(const int2* __restrict__ hash, int W, int H, int k, int ko, int u0,
+                                             int v0, int tw, int th, int* s_sum) {
+    int ra, va, RW, RH;
+    tile_region(W, H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
+    if (threadIdx.x == 0) *s_sum = 0;
+    __syncthreads();
+    for (int r = threadIdx.x; r < RH; r += blockDim.x) {
+        int lo = 0, hi = 0;
+        bool found = false;
+        const int2* row = hash + (int64_t)(va + r) * W + ra;
+        for (int c = 0; c < RW; ++c) {
+            const int2 e = row[c];
+            if (e.y > 0) {
+                if (!found) lo = e.x;
+                found = true;
+                hi = e.x + e.y;
+            }
+        }
+        atomicAdd(s_sum, hi - lo);
+    }
+    __syncthreads();
+    return *s_sum;
+}
+
+// Stages the halo region of the tile with top-left ray (u0, v0) into LDS.
+// Returns the number of staged points, or -1 if the region holds more than
+// `cap` points (the tile then reads global memory).  Block-wide; every
+// thread of the block must call it.
+__device__ __forceinline__ int stage_tile(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
+                                          const float* __restrict__ xyz, int W, int H, int k, int ko, int u0,
+                                          int v0, int tw, int th, int cap, float4* s_pts, int2* s_hash,
+                                          int* s_rowlo, int* s_rowbase, int* s_total, int& ra, int& va,
+                                          int& RW) {
+    int RH;
+    tile_region(W, H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    for (int i = tid; i < RW * RH; i += nthr) s_hash[i] = hash[(int64_t)(va + i / RW) * W + ra + i % RW];
+    __syncthreads();
+    for (int r = tid; r < RH; r += nthr) {
+        int lo = 0, hi = 0;
+        bool found = false;
+        for (int c = 0; c < RW; ++c) {
+            const int2 e = s_hash[r * RW + c];
+            if (e.y > 0) {
+                if (!found) lo = e.x;
+                found = true;
+                hi = e.x + e.y;
+            }
+        }
+        s_rowlo[r] = lo;
+        s_rowbase[r] = hi - lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int r = 0; r < RH; ++r) {
+            const int len = s_rowbase[r];
+            s_rowbase[r] = run;
+            run += len;
+        }
+        s_rowbase[RH] = run;
+        *s_total = run;
+    }
+    __syncthreads();
+    const int total = *s_total;
+    if (total > cap) return -1;
+    for (int i = tid; i < total; i += nthr) {
+        int lo = 0, hi = RH;  // last row r with rowbase[r] <= i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_rowbase[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const int gi = s_rowlo[lo] + (i - s_rowbase[lo]);
+        const int id = sorted_idx[gi];
+        const float* p = xyz + 3 * (int64_t)id;
+        s_pts[i] = make_float4(p[0], p[1], p[2], __int_as_float(id));
+    }
+    for (int i = tid; i < RW * RH; i += nthr) {
+        const int2 e = s_hash[i];
+        if (e.y > 0) {
+            const int r = i / RW;
+            s_hash[i].x = s_rowbase[r] + (e.x - s_rowlo[r]);
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+}  // namespace epcq

@@ -56,6 +56,15 @@ class _Options(ctypes.Structure):
     ]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("tile_cap", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+TUNE_XCD_ORDER = 1
+TUNE_PREFILTER = 2
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -66,6 +75,9 @@ _SZ = ctypes.c_size_t
 ABI_SYMBOLS = {
     "epcq_abi_version": (_I32, []),
     "epcq_last_error": (ctypes.c_char_p, []),
+    "epcq_set_tuning": (_I32, [ctypes.POINTER(_Tuning)]),
+    "epcq_get_tuning": (_I32, [ctypes.POINTER(_Tuning)]),
+    "epcq_squared_threshold": (_F32, [_F32]),
     "epcq_scatter_hashtable": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
     "epcq_scatter": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P]),
     "epcq_build_hashtable_workspace_size": (_SZ, [_I64, _I32, _I32]),
@@ -393,6 +405,23 @@ def quick_query_for_nearby_point(neural_points, cam, options, sorted_point_idx_l
                                         _ptr(sorted_point_idx_list), _ptr(hashtable), _ptr(indices_of_ray), R,
                                         _ptr(out), _stream(device)))
     return out
+
+
+def set_tuning(tile_w=16, tile_h=16, tile_cap=3000, flags=0):
+    """Launch shape of the full-image samplers: tile_w x tile_h rays per
+    block, tile_cap points staged in LDS per tile, TUNE_* flags."""
+    _raise_on(_lib.epcq_set_tuning(ctypes.byref(_Tuning(int(tile_w), int(tile_h), int(tile_cap), int(flags)))))
+
+
+def get_tuning():
+    t = _Tuning()
+    _raise_on(_lib.epcq_get_tuning(ctypes.byref(t)))
+    return {"tile_w": t.tile_w, "tile_h": t.tile_h, "tile_cap": t.tile_cap, "flags": t.flags}
+
+
+def squared_threshold(r):
+    """Smallest float s with sqrtf(s) >= r."""
+    return float(_lib.epcq_squared_threshold(float(r)))
 
 
 def max_live_samples(options, r_max):

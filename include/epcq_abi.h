@@ -68,6 +68,30 @@ int epcq_abi_version(void);
 const char* epcq_last_error(void);
 
 /*
+ * Launch shape of the full-image samplers (no counterpart in the reference,
+ * whose launch shape is fixed).  One block samples a tile_w x tile_h tile of
+ * rays (tile_w * tile_h in {64, 128, 256}, tile_w a power of two <= 64);
+ * tile_cap = points of the tile's halo region staged in LDS (clamped to the
+ * 64 KB block limit).  flags: EPCQ_TUNE_XCD_ORDER gives each XCD a
+ * contiguous run of tiles; EPCQ_TUNE_PREFILTER enables the cross-product
+ * rejection before the exact point-to-ray test.  Process-wide, guarded by a
+ * mutex; the autotuner sets it.  Defaults: 16, 16, 3000, 0.
+ */
+typedef struct epcq_tuning {
+    int32_t tile_w;
+    int32_t tile_h;
+    int32_t tile_cap;
+    int32_t flags;
+} epcq_tuning;
+#define EPCQ_TUNE_XCD_ORDER 1
+#define EPCQ_TUNE_PREFILTER 2
+int epcq_set_tuning(const epcq_tuning* t);
+int epcq_get_tuning(epcq_tuning* t);
+
+/* Smallest float s with sqrtf(s) >= r (host helper; sqrtf(e2) < r <=> e2 < s). */
+float epcq_squared_threshold(float r);
+
+/*
  * scatter_hashtable -- replaces `scatter_hashtable`
  * (epcq/csrc/epcq.cpp:6, epcq/csrc/rasterization.cu:130-171).
  * Inputs are pixel-sorted (keys ascending, off-image key = H*W) as produced

@@ -62,7 +62,8 @@ def test_build_edge_cases(dev):
     hot = np.zeros((60_000, 3), np.float32)
     hot[:, 1] = np.linspace(-0.5, 0.5, 60_000, dtype=np.float32)  # all on the optical axis
     gi, gd, gh = check_build(hot, cam, dev, sort=True)
-    assert gh[:, 1].max() == 60_000
+    busy = gh[:, 1] > 0
+    assert gh[busy, 1].sum() == 60_000 and busy.sum() <= 4 and gh[:, 1].max() > 20_000
     print("build edge: hot pixel + background", flush=True)
     mix = np.concatenate([hot[:20_000], synth.sphere_cloud(50_000, seed=3)])
     check_build(mix, cam, dev, sort=False)
