@@ -1,0 +1,409 @@
+// Device bodies and launch plan of the fused hash-table build, shared by the
+// per-view entry point (epcq_build.hip) and the batched multi-view kernels
+// (epcq_batch.hip).  See epcq_build.hip for the algorithm.
+#pragma once
+
+#include <algorithm>
+
+#include "epcq_common.h"
+#include "epcq_device.h"
+
+namespace epcq {
+
+constexpr int kBuildBlock = 256;
+constexpr int kBuildWaves = kBuildBlock / kWave;
+constexpr int kMaxBinBits = 13;    // <= 8192 coarse bins (32 KB of LDS cursors)
+constexpr int kMinTileShift = 10;  // >= 1024 pixels per bin
+constexpr int kSegBlocks = 64;     // blocks per column-scan segment
+
+struct ProjParams {
+    float w[12];
+    float K[9];
+    int H, W;
+    int P;
+};
+
+__device__ __forceinline__ void load_proj(const float* __restrict__ w2c, const float* __restrict__ K,
+                                          int H, int W, ProjParams& pp) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) pp.w[i] = w2c[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pp.K[i] = K[i];
+    pp.H = H;
+    pp.W = W;
+    pp.P = H * W;
+}
+
+// _world2cam / _cam2image / _convetpixel (epcq/epcq.py:251-274): camera depth
+// and row-major pixel key, P for points whose truncated screen coordinate
+// falls outside the image.
+__device__ __forceinline__ int project_key(const ProjParams& pp, float x, float y, float z, float& depth) {
+    const float* w = pp.w;
+    const float* K = pp.K;
+    float X = w[0] * x;
+    X = X + w[1] * y;
+    X = X + w[2] * z;
+    X = X + w[3];
+    float Y = w[4] * x;
+    Y = Y + w[5] * y;
+    Y = Y + w[6] * z;
+    Y = Y + w[7];
+    float Z = w[8] * x;
+    Z = Z + w[9] * y;
+    Z = Z + w[10] * z;
+    Z = Z + w[11];
+    float sx = K[0] * X;
+    sx = sx + K[1] * Y;
+    sx = sx + K[2] * Z;
+    float sy = K[3] * X;
+    sy = sy + K[4] * Y;
+    sy = sy + K[5] * Z;
+    float sz = K[6] * X;
+    sz = sz + K[7] * Y;
+    sz = sz + K[8] * Z;
+    sx = sx / sz;
+    sy = sy / sz;
+    depth = Z;
+    // trunc-toward-zero then 0 <= u < W, 0 <= v < H, without converting
+    // out-of-range values (NaN fails every comparison)
+    const bool in = (sx > -1.0f) && (sx < (float)pp.W) && (sy > -1.0f) && (sy < (float)pp.H);
+    return in ? ((int)sx + (int)sy * pp.W) : pp.P;
+}
+
+// Exclusive scan of one int per thread over the 256-thread block.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& total) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / kWave;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) s_wave[wave] = x;
+    __syncthreads();
+    int wbase = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kBuildWaves; ++w) {
+        const int t = s_wave[w];
+        if (w < wave) wbase += t;
+        total += t;
+    }
+    __syncthreads();
+    return wbase + x - v;
+}
+
+// Exclusive scan of a[0..len) in LDS (len a multiple of 256), in place.
+__device__ void lds_exclusive_scan(int* a, int len, int* s_wave) {
+    const int per = len / kBuildBlock;
+    int* seg = a + threadIdx.x * per;
+    int sum = 0;
+    for (int j = 0; j < per; ++j) sum += seg[j];
+    int total;
+    int run = block_exclusive_scan(sum, s_wave, total);
+    for (int j = 0; j < per; ++j) {
+        const int v = seg[j];
+        seg[j] = run;
+        run += v;
+    }
+    __syncthreads();
+}
+
+// Appends this round's keyed elements to per-key cursors in LDS in lane order
+// within a wave and wave order within the block; returns each active lane's
+// position (undefined for inactive lanes).
+__device__ __forceinline__ int ordered_append(int* cursor, uint32_t key, int bits, bool active) {
+    const uint64_t m = match_lanes(key, bits, active);
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / kWave;
+    const int leader = m ? (__ffsll((unsigned long long)m) - 1) : 0;
+    const int rank = __popcll(m & lanemask_lt());
+    int base = 0;
+    for (int w = 0; w < kBuildWaves; ++w) {
+        if (wave == w && active && lane == leader) {
+            base = cursor[key];
+            cursor[key] = base + __popcll(m);
+        }
+        __syncthreads();
+    }
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
+
+// Everything one view's build reads and writes.
+struct BuildView {
+    const float* xyz;
+    int64_t n;
+    const float* w2c;  // >= 3 rows x 4, row-major
+    const float* K;    // 3 x 3, row-major
+    int H, W, P;
+    int ts, nbins, bin_bits;
+    int64_t chunk;
+    int nblk, nseg;
+    int sort;
+    int* hist;    // [nblk][nbins] counts, then write cursors
+    int* part;    // [nseg][nbins]
+    int* bstart;  // [nbins + 1]
+    int* btotal;  // [nbins]
+    int* tidx;    // [n] binned point index
+    float* tdep;  // [n] binned depth
+    uint16_t* tfine;  // [n] pixel within bin
+    int* out_idx;     // [n]
+    float* out_depth; // [n] (may be null: not written)
+    int* out_hash;    // [P, 2]
+};
+
+// Launch geometry of the build for (n, H, W) and its workspace layout.
+struct BuildPlan {
+    int64_t P;
+    int ts;         // log2 pixels per coarse bin
+    int nbins;
+    int bin_bits;   // bits of a coarse bin id
+    int64_t chunk;  // points per block in count / bin
+    int nblk;
+    int nseg;
+    size_t off_hist, off_part, off_start, off_total, off_tidx, off_tdep, off_tfine, bytes;
+};
+
+inline BuildPlan plan_build(int64_t n, int H, int W) {
+    BuildPlan bp{};
+    bp.P = (int64_t)H * W;
+    bp.ts = std::max(kMinTileShift, ceil_log2(bp.P) - kMaxBinBits);
+    bp.nbins = (int)ceil_div(bp.P, int64_t(1) << bp.ts);
+    bp.bin_bits = std::max(1, ceil_log2(bp.nbins));
+    // about 2048 blocks at most, each a multiple of 4096 points
+    const int64_t unit = 4096;
+    const int64_t units = std::max<int64_t>(1, ceil_div(ceil_div(n, unit), 2048));
+    bp.chunk = unit * units;
+    bp.nblk = (int)std::max<int64_t>(1, ceil_div(n, bp.chunk));
+    bp.nseg = (int)ceil_div(bp.nblk, kSegBlocks);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = align_up(off + bytes, 256);
+        return at;
+    };
+    bp.off_hist = take((size_t)bp.nblk * bp.nbins * 4);
+    bp.off_part = take((size_t)bp.nseg * bp.nbins * 4);
+    bp.off_start = take((size_t)(bp.nbins + 1) * 4);
+    bp.off_total = take((size_t)bp.nbins * 4);
+    bp.off_tidx = take((size_t)n * 4);
+    bp.off_tdep = take((size_t)n * 4);
+    bp.off_tfine = take((size_t)n * 2);
+    bp.bytes = off;
+    return bp;
+}
+
+// BuildView of plan bp over workspace ws.
+inline BuildView make_build_view(const BuildPlan& bp, const float* xyz, int64_t n, const float* w2c,
+                                 const float* K, int H, int W, int sort, void* ws, int* out_idx,
+                                 float* out_depth, int* out_hash) {
+    char* w = static_cast<char*>(ws);
+    BuildView v{};
+    v.xyz = xyz;
+    v.n = n;
+    v.w2c = w2c;
+    v.K = K;
+    v.H = H;
+    v.W = W;
+    v.P = (int)bp.P;
+    v.ts = bp.ts;
+    v.nbins = bp.nbins;
+    v.bin_bits = bp.bin_bits;
+    v.chunk = bp.chunk;
+    v.nblk = bp.nblk;
+    v.nseg = bp.nseg;
+    v.sort = sort;
+    v.hist = reinterpret_cast<int*>(w + bp.off_hist);
+    v.part = reinterpret_cast<int*>(w + bp.off_part);
+    v.bstart = reinterpret_cast<int*>(w + bp.off_start);
+    v.btotal = reinterpret_cast<int*>(w + bp.off_total);
+    v.tidx = reinterpret_cast<int*>(w + bp.off_tidx);
+    v.tdep = reinterpret_cast<float*>(w + bp.off_tdep);
+    v.tfine = reinterpret_cast<uint16_t*>(w + bp.off_tfine);
+    v.out_idx = out_idx;
+    v.out_depth = out_depth;
+    v.out_hash = out_hash;
+    return v;
+}
+
+// ---- 1. count: block `blk` histograms its chunk over the coarse bins --------
+__device__ __forceinline__ void count_body(const BuildView& bv, int blk, int* s_hist) {
+    ProjParams pp;
+    load_proj(bv.w2c, bv.K, bv.H, bv.W, pp);
+    for (int b = threadIdx.x; b < bv.nbins; b += kBuildBlock) s_hist[b] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blk * bv.chunk;
+    const int64_t end = min(base + bv.chunk, bv.n);
+    const float* xyz = bv.xyz;
+#pragma unroll 4
+    for (int64_t i = base + threadIdx.x; i < end; i += kBuildBlock) {
+        float depth;
+        const int key = project_key(pp, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], depth);
+        if (key < pp.P) atomicAdd(&s_hist[key >> bv.ts], 1);
+    }
+    __syncthreads();
+    int* row = bv.hist + (int64_t)blk * bv.nbins;
+    for (int b = threadIdx.x; b < bv.nbins; b += kBuildBlock) row[b] = s_hist[b];
+    __syncthreads();
+}
+
+// ---- 2. scan -------------------------------------------------------------------
+// part[seg][b] = sum of hist[blk][b] over the blocks of segment seg.
+__device__ __forceinline__ void col_partial_body(const BuildView& bv, int b, int seg) {
+    if (b >= bv.nbins) return;
+    const int b0 = seg * kSegBlocks, b1 = min(b0 + kSegBlocks, bv.nblk);
+    int s = 0;
+    for (int blk = b0; blk < b1; ++blk) s += bv.hist[(int64_t)blk * bv.nbins + b];
+    bv.part[(int64_t)seg * bv.nbins + b] = s;
+}
+
+// One block: bin totals, bin starts (exclusive scan over bins, N_in at the
+// end) and absolute segment offsets part[seg][b].
+__device__ __forceinline__ void bin_scan_body(const BuildView& bv, int* s_wave, int* s_carry) {
+    const int nbins = bv.nbins, nseg = bv.nseg;
+    if (threadIdx.x == 0) *s_carry = 0;
+    for (int b0 = 0; b0 < nbins; b0 += kBuildBlock) {
+        const int b = b0 + threadIdx.x;
+        int tot = 0;
+        if (b < nbins) {
+            for (int seg = 0; seg < nseg; ++seg) {
+                const int v = bv.part[(int64_t)seg * nbins + b];
+                bv.part[(int64_t)seg * nbins + b] = tot;
+                tot += v;
+            }
+        }
+        __syncthreads();
+        const int carry = *s_carry;
+        int total;
+        const int ex = block_exclusive_scan(tot, s_wave, total);
+        if (b < nbins) {
+            const int start = carry + ex;
+            bv.bstart[b] = start;
+            bv.btotal[b] = tot;
+            for (int seg = 0; seg < nseg; ++seg) bv.part[(int64_t)seg * nbins + b] += start;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *s_carry = carry + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bv.bstart[nbins] = *s_carry;
+    __syncthreads();
+}
+
+// hist[blk][b] <- absolute write cursor of block blk in bin b.
+__device__ __forceinline__ void col_apply_body(const BuildView& bv, int b, int seg) {
+    if (b >= bv.nbins) return;
+    const int b0 = seg * kSegBlocks, b1 = min(b0 + kSegBlocks, bv.nblk);
+    int run = bv.part[(int64_t)seg * bv.nbins + b];
+    for (int blk = b0; blk < b1; ++blk) {
+        const int64_t at = (int64_t)blk * bv.nbins + b;
+        const int v = bv.hist[at];
+        bv.hist[at] = run;
+        run += v;
+    }
+}
+
+// ---- 3. bin: block `blk` appends its chunk's points to their bins -------------
+__device__ __forceinline__ void bin_body(const BuildView& bv, int blk, int* s_cur) {
+    ProjParams pp;
+    load_proj(bv.w2c, bv.K, bv.H, bv.W, pp);
+    const int* row = bv.hist + (int64_t)blk * bv.nbins;
+    for (int b = threadIdx.x; b < bv.nbins; b += kBuildBlock) s_cur[b] = row[b];
+    __syncthreads();
+    const int64_t base = (int64_t)blk * bv.chunk;
+    const int64_t end = min(base + bv.chunk, bv.n);
+    const uint32_t fmask = (1u << bv.ts) - 1u;
+    const float* xyz = bv.xyz;
+    for (int64_t r0 = base; r0 < end; r0 += kBuildBlock) {
+        const int64_t i = r0 + threadIdx.x;
+        float depth = 0.0f;
+        int key = pp.P;
+        if (i < end) key = project_key(pp, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], depth);
+        const bool active = key < pp.P;
+        const uint32_t bin = active ? ((uint32_t)key >> bv.ts) : 0u;
+        const int pos = ordered_append(s_cur, bin, bv.bin_bits, active);
+        if (active) {
+            bv.tidx[pos] = (int)i;
+            bv.tdep[pos] = depth;
+            bv.tfine[pos] = (uint16_t)((uint32_t)key & fmask);
+        }
+    }
+    __syncthreads();
+}
+
+// ---- 4. fine: bin `b` of the view: pixel counts, hash entries, stable
+// placement, optional depth sort, and its share of the off-image tail ----------
+__device__ __forceinline__ void fine_body(const BuildView& bv, int b, int* s_mem, int* s_wave) {
+    const int ts = bv.ts;
+    const int tile = 1 << ts;
+    int* s_cnt = s_mem;
+    int* s_cur = s_mem + tile;
+    const int bs = bv.bstart[b];
+    const int tot = bv.btotal[b];
+    for (int f = threadIdx.x; f < tile; f += kBuildBlock) s_cnt[f] = 0;
+    __syncthreads();
+    for (int i = bs + threadIdx.x; i < bs + tot; i += kBuildBlock) atomicAdd(&s_cnt[bv.tfine[i]], 1);
+    __syncthreads();
+    for (int f = threadIdx.x; f < tile; f += kBuildBlock) s_cur[f] = s_cnt[f];
+    __syncthreads();
+    lds_exclusive_scan(s_cur, tile, s_wave);
+    const int64_t p0 = (int64_t)b << ts;
+    for (int f = threadIdx.x; f < tile; f += kBuildBlock) {
+        const int64_t p = p0 + f;
+        if (p < bv.P) {
+            const int c = s_cnt[f];
+            bv.out_hash[2 * p] = c ? bs + s_cur[f] : -1;
+            bv.out_hash[2 * p + 1] = c ? c : -1;
+        }
+    }
+    __syncthreads();
+    float* out_depth = bv.out_depth;
+    for (int r0 = bs; r0 < bs + tot; r0 += kBuildBlock) {
+        const int i = r0 + threadIdx.x;
+        const bool active = i < bs + tot;
+        const uint32_t f = active ? bv.tfine[i] : 0u;
+        const int pos = ordered_append(s_cur, f, ts, active);
+        if (active) {
+            bv.out_idx[bs + pos] = bv.tidx[i];
+            if (out_depth) out_depth[bs + pos] = bv.tdep[i];
+        }
+    }
+    if (bv.sort && out_depth) {
+        __syncthreads();
+        // stable insertion sort of each pixel run by depth (same result as the
+        // reference's stable bubble sort, cuda_util.cuh:291-313)
+        for (int f = threadIdx.x; f < tile; f += kBuildBlock) {
+            const int c = s_cnt[f];
+            if (c < 2) continue;
+            const int s = bs + s_cur[f] - c;
+            float* d = out_depth + s;
+            int* ix = bv.out_idx + s;
+            for (int a = 1; a < c; ++a) {
+                const float kd = d[a];
+                const int ki = ix[a];
+                int j = a - 1;
+                while (j >= 0 && d[j] > kd) {
+                    d[j + 1] = d[j];
+                    ix[j + 1] = ix[j];
+                    --j;
+                }
+                d[j + 1] = kd;
+                ix[j + 1] = ki;
+            }
+        }
+    }
+    // off-image tail: positions [N_in, n) hold -1 like the reference's
+    // torch::full(-1) outputs; bin b fills every nbins-th run of 256
+    const int64_t n_in = bv.bstart[bv.nbins];
+    for (int64_t i = n_in + (int64_t)b * kBuildBlock + threadIdx.x; i < bv.n;
+         i += (int64_t)bv.nbins * kBuildBlock) {
+        bv.out_idx[i] = -1;
+        if (out_depth) out_depth[i] = -1.0f;
+    }
+    __syncthreads();
+}
+
+}  // namespace epcq

@@ -45,7 +45,8 @@ TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags) {
     p.xcd = (flags & EPCQ_TUNE_XCD_ORDER) != 0;
     p.rw_max = p.tw + k - 1;
     p.rh_max = p.th + k - 1;
-    const size_t fixed = align_up((size_t)p.rw_max * p.rh_max * 8 + (size_t)(2 * p.rh_max + 1) * 4, 16);
+    const size_t fixed =
+        align_up((size_t)p.rw_max * p.rh_max * (16 + 8) + (size_t)(2 * p.rh_max + 1) * 4, 16);
     const size_t limit = 64 * 1024;
     size_t c = (size_t)std::max(cap, 0);
     if (fixed + c * 16 > limit) c = fixed < limit ? (limit - fixed) / 16 : 0;
@@ -67,46 +68,14 @@ epcq_tuning current_tuning() {
     return g_tuning;
 }
 
-struct ViewArgs {
-    const float* c2w;
-    float fx, fy, cx, cy;
-    int W, H;
-    const int2* hash;
-    const int* sorted_idx;
-};
-
-__device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
-    float m[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) m[i] = v.c2w[i];
-    make_ray(m, v.fx, v.fy, v.cx, v.cy, u, vv, ray);
-}
-
 template <int MODE, int KMAX, int SMAX>
 __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
                                                              int tiles_x) {
     extern __shared__ float4 s_dyn[];
     __shared__ int s_total;
-    float4* s_pts = s_dyn;
-    int2* s_hash = reinterpret_cast<int2*>(s_dyn + tp.cap);
-    int* s_rowlo = reinterpret_cast<int*>(s_hash + tp.rw_max * tp.rh_max);
-    int* s_rowbase = s_rowlo + tp.rh_max;
-
+    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total);
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int u0 = (tile % tiles_x) * tp.tw;
-    const int v0 = (tile / tiles_x) * tp.th;
-    int ra, va, RW;
-    const int staged = stage_tile(view.hash, view.sorted_idx, o.xyz, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th,
-                                  tp.cap, s_pts, s_hash, s_rowlo, s_rowbase, &s_total, ra, va, RW);
-    if (staged < 0) return;  // region larger than the LDS capacity: sample_tile_fallback_kernel
-    const int u = u0 + (int)threadIdx.x % tp.tw;
-    const int v = v0 + (int)threadIdx.x / tp.tw;
-    if (u >= view.W || v >= view.H) return;
-    const int64_t ri = (int64_t)v * view.W + u;
-    Ray ray;
-    view_ray(view, u, v, ray);
-    const SrcLds src{s_hash, s_pts, ra, va, RW};
-    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, true);
+    sample_tile_body<MODE, KMAX, SMAX>(o, out, view, tp, tile, tiles_x, 0, sm);
 }
 
 // The tiles sample_tile_kernel skipped (halo region above the LDS capacity),
@@ -116,18 +85,7 @@ __global__ __launch_bounds__(kBlock) void sample_tile_fallback_kernel(SampleOpts
                                                                       TilePlan tp, int tiles_x) {
     __shared__ int s_sum;
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int u0 = (tile % tiles_x) * tp.tw;
-    const int v0 = (tile / tiles_x) * tp.th;
-    const int total = region_points(view.hash, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, &s_sum);
-    if (total <= tp.cap) return;
-    const int u = u0 + (int)threadIdx.x % tp.tw;
-    const int v = v0 + (int)threadIdx.x / tp.tw;
-    if (u >= view.W || v >= view.H) return;
-    const int64_t ri = (int64_t)v * view.W + u;
-    Ray ray;
-    view_ray(view, u, v, ray);
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.W};
-    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, true);
+    sample_tile_fallback_body<MODE, KMAX, SMAX>(o, out, view, tp, tile, tiles_x, 0, &s_sum);
 }
 
 template <int MODE, int KMAX, int SMAX>
@@ -140,8 +98,17 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
     Ray ray;
     view_ray(view, u, v, ray);
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.W};
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.spheres, view.W};
     run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
+}
+
+__global__ __launch_bounds__(kBlock) void bucket_bounds_kernel(const int2* __restrict__ hash,
+                                                               const int* __restrict__ sorted_idx,
+                                                               const float* __restrict__ xyz,
+                                                               const float* __restrict__ radius, float r0, int64_t P,
+                                                               float4* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < P) bucket_bounds_body(hash, sorted_idx, xyz, radius, r0, p, out);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each
@@ -259,9 +226,21 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     return o;
 }
 
-ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash) {
+ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash, const float4* spheres) {
     return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
-                    reinterpret_cast<const int2*>(hash), sorted_idx};
+                    reinterpret_cast<const int2*>(hash), sorted_idx, spheres};
+}
+
+// Bucket bounds into the workspace (when it is large enough); returns the
+// sphere array or nullptr (no culling).
+const float4* prepare_bounds(const SampleOpts& o, const epcq_camera* cam, const int* sorted_idx, const int* hash,
+                             void* workspace, size_t workspace_bytes, hipStream_t st) {
+    const int64_t P = (int64_t)cam->width * cam->height;
+    if (!workspace || workspace_bytes < (size_t)P * sizeof(float4)) return nullptr;
+    float4* sph = static_cast<float4*>(workspace);
+    bucket_bounds_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
+        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, P, sph);
+    return sph;
 }
 
 template <int MODE, int KMAX, int SMAX>
@@ -274,6 +253,7 @@ void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
         const int tiles_x = (int)ceil_div(view.W, tp.tw);
         const int tiles = tiles_x * (int)ceil_div(view.H, tp.th);
         const dim3 block(tp.tw * tp.th);
+        (void)kBlock;
         sample_tile_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp, tiles_x);
         sample_tile_fallback_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, 0, st>>>(o, out, view, tp, tiles_x);
     }
@@ -359,9 +339,15 @@ extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max) {
     return visits;
 }
 
+extern "C" size_t epcq_sampling_workspace_size(int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0) return 0;
+    return (size_t)height * width * sizeof(float4);
+}
+
 extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
                                    const int32_t* sorted_idx, const int32_t* hash, int32_t mode, float* sp_xyz,
-                                   int32_t* pc_idx, float* sp_t, uint8_t* mask, void* stream) {
+                                   int32_t* pc_idx, float* sp_t, uint8_t* mask, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
     if (int e = check_common(cam, opt, "quick_sampling")) return e;
     if (mode != EPCQ_MODE_COMPAT && mode != EPCQ_MODE_KNN)
         return fail(EPCQ_ERR_INVALID, "quick_sampling: unknown mode");
@@ -373,21 +359,25 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
     if (opt->num_sample_points_per_ray == 0) return EPCQ_OK;
     int bound;
     if (int e = live_bound(opt, opt->radius_threshold, "quick_sampling", &bound)) return e;
-    const SampleOpts o = make_opts(xyz, opt, nullptr);
+    SampleOpts o = make_opts(xyz, opt, nullptr);
     SampleOut out{};
     out.sp_xyz = sp_xyz;
     out.pc_idx = pc_idx;
     out.sp_t = sp_t;
     out.mask = mask;
-    return launch_sample(o, out, make_view(cam, sorted_idx, hash), nullptr, (int64_t)cam->width * cam->height, mode,
-                         bound, as_stream(stream), "quick_sampling");
+    hipStream_t st = as_stream(stream);
+    const float4* sph = prepare_bounds(o, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    o.cull = sph != nullptr;
+    return launch_sample(o, out, make_view(cam, sorted_idx, hash, sph), nullptr, (int64_t)cam->width * cam->height,
+                         mode, bound, st, "quick_sampling");
 }
 
 extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
                                  const int32_t* sorted_idx, const int32_t* hash, const float* radius,
                                  const int32_t* rays, int64_t R, int32_t mode, int32_t s_max, int32_t* n_samples,
                                  float* sample_t, int32_t* nb_idx, float* nb_t, float* nb_dist, const void* feat,
-                                 int32_t feat_bf16, int32_t F, float* w, float* agg, void* stream) {
+                                 int32_t feat_bf16, int32_t F, float* w, float* agg, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
     if (int e = check_common(cam, opt, "knn_sampling")) return e;
     if (mode != EPCQ_MODE_COMPAT && mode != EPCQ_MODE_KNN)
         return fail(EPCQ_ERR_INVALID, "knn_sampling: unknown mode");
@@ -420,8 +410,10 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     out.w = w;
     out.agg = agg;
     const int64_t nr = rays ? R : (int64_t)cam->width * cam->height;
-    return launch_sample(o, out, make_view(cam, sorted_idx, hash), rays, nr, mode, bound, as_stream(stream),
-                         "knn_sampling");
+    hipStream_t st = as_stream(stream);
+    const float4* sph = prepare_bounds(o, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    o.cull = sph != nullptr;
+    return launch_sample(o, out, make_view(cam, sorted_idx, hash, sph), rays, nr, mode, bound, st, "knn_sampling");
 }
 
 extern "C" int epcq_quick_query(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,

@@ -53,6 +53,7 @@ struct SampleOpts {
     const void* feat;
     int feat_bf16, F;
     int prefilter;        // cross-product rejection before the exact test
+    int cull;             // skip neighbour buckets by their bounding spheres
 };
 
 struct SampleOut {
@@ -76,8 +77,10 @@ struct SrcGlobal {
     const int2* hash;
     const int* sorted_idx;
     const float* xyz;
+    const float4* spheres;  // per-pixel bucket bounds (when culling)
     int W;
     __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[vb * W + ub]; }
+    __device__ __forceinline__ float4 sphere(int ub, int vb) const { return spheres[vb * W + ub]; }
     __device__ __forceinline__ float4 point(int j) const {
         const int id = sorted_idx[j];
         const float* p = xyz + 3 * (int64_t)id;
@@ -88,8 +91,10 @@ struct SrcGlobal {
 struct SrcLds {
     const int2* hash;
     const float4* pts;
+    const float4* spheres;
     int ra, va, RW;
     __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float4 sphere(int ub, int vb) const { return spheres[(vb - va) * RW + (ub - ra)]; }
     __device__ __forceinline__ float4 point(int j) const { return pts[j]; }
 };
 
@@ -118,6 +123,31 @@ __device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, f
     const float c2 = __builtin_fmaf(w0, ray.d[1], -(w1 * ray.d[0]));
     const float cs = __builtin_fmaf(c2, c2, __builtin_fmaf(c1, c1, c0 * c0));
     return cs > rm * rm * ray.dd;
+}
+
+// Bucket culling.  sph = (c, B): every point p of the bucket has |p - c| <= R
+// and radius <= B - R.  The distance from p to the ray (or to a point on it)
+// is at least the distance from c minus R, and the canonical rounded
+// distances differ from the exact ones by far less than the margin
+// 3e-6 (1 + |c - o|_1 + 3 B + |o|_1), so a bucket is skipped only if none of
+// its points can pass the exact test.
+__device__ __forceinline__ bool bucket_far_from_ray(const Ray& ray, const float4& sph) {
+    const float w0 = sph.x - ray.o[0], w1 = sph.y - ray.o[1], w2 = sph.z - ray.o[2];
+    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
+    const float lim = sph.w + __builtin_fmaf(3e-6f, aw + 3.0f * sph.w, ray.omargin);
+    const float c0 = __builtin_fmaf(w1, ray.d[2], -(w2 * ray.d[1]));
+    const float c1 = __builtin_fmaf(w2, ray.d[0], -(w0 * ray.d[2]));
+    const float c2 = __builtin_fmaf(w0, ray.d[1], -(w1 * ray.d[0]));
+    const float cs = __builtin_fmaf(c2, c2, __builtin_fmaf(c1, c1, c0 * c0));
+    return cs > lim * lim * ray.dd;
+}
+
+__device__ __forceinline__ bool bucket_far_from_point(const float* sp, const float4& sph) {
+    const float w0 = sph.x - sp[0], w1 = sph.y - sp[1], w2 = sph.z - sp[2];
+    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
+    const float lim = sph.w + 3e-6f * (1.0f + aw + 3.0f * sph.w);
+    const float d2 = __builtin_fmaf(w2, w2, __builtin_fmaf(w1, w1, w0 * w0));
+    return d2 > lim * lim;
 }
 
 // dist < radius for a squared distance, exact (see the file comment).
@@ -150,6 +180,7 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
     for_each_lane(o, u, v, W, H, [&](int ub, int vb) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
+        if (o.cull && bucket_far_from_ray(ray, src.sphere(ub, vb))) return;
         int kept = 0;
         float4 nxt = src.point(e.x);
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
@@ -299,6 +330,14 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
     auto bucket = [&](int ub, int vb) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
+        if (o.cull) {
+            const float4 sph = src.sphere(ub, vb);
+            bool far = true;
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+                if (q < ns && !bucket_far_from_point(sp[q], sph)) far = false;
+            if (far) return;
+        }
         float4 nxt = src.point(e.x);
         for (int j = 0; j < e.y; ++j) {
             const float4 p = nxt;
@@ -630,6 +669,44 @@ struct TilePlan {
     size_t lds_bytes;
 };
 
+// Geometry and inputs of one view for the samplers.
+struct ViewArgs {
+    const float* c2w;  // device, >= 3 rows x 4
+    float fx, fy, cx, cy;
+    int W, H;
+    const int2* hash;
+    const int* sorted_idx;
+    const float4* spheres;  // bucket bounds or nullptr
+};
+
+__device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
+    float m[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) m[i] = v.c2w[i];
+    make_ray(m, v.fx, v.fy, v.cx, v.cy, u, vv, ray);
+}
+
+// LDS of one tile: points [cap] | spheres [rw*rh] | hash [rw*rh] | rowlo | rowbase.
+struct TileSmem {
+    float4* pts;
+    float4* sph;
+    int2* hash;
+    int* rowlo;
+    int* rowbase;
+    int* total;
+};
+
+__device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan& tp, int* total) {
+    TileSmem m;
+    m.pts = dyn;
+    m.sph = dyn + tp.cap;
+    m.hash = reinterpret_cast<int2*>(m.sph + tp.rw_max * tp.rh_max);
+    m.rowlo = reinterpret_cast<int*>(m.hash + tp.rw_max * tp.rh_max);
+    m.rowbase = m.rowlo + tp.rh_max;
+    m.total = total;
+    return m;
+}
+
 TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags);
 
 // Halo region of the tile with top-left ray (u0, v0): pixels [ra, ra+RW) x
@@ -675,15 +752,19 @@ __device__ __forceinline__ int region_points(const int2* __restrict__ hash, int 
 // `cap` points (the tile then reads global memory).  Block-wide; every
 // thread of the block must call it.
 __device__ __forceinline__ int stage_tile(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
-                                          const float* __restrict__ xyz, int W, int H, int k, int ko, int u0,
-                                          int v0, int tw, int th, int cap, float4* s_pts, int2* s_hash,
-                                          int* s_rowlo, int* s_rowbase, int* s_total, int& ra, int& va,
-                                          int& RW) {
+                                          const float* __restrict__ xyz, const float4* __restrict__ spheres,
+                                          int W, int H, int k, int ko, int u0, int v0, int tw, int th, int cap,
+                                          float4* s_pts, int2* s_hash, float4* s_sph, int* s_rowlo,
+                                          int* s_rowbase, int* s_total, int& ra, int& va, int& RW) {
     int RH;
     tile_region(W, H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    for (int i = tid; i < RW * RH; i += nthr) s_hash[i] = hash[(int64_t)(va + i / RW) * W + ra + i % RW];
+    for (int i = tid; i < RW * RH; i += nthr) {
+        const int64_t g = (int64_t)(va + i / RW) * W + ra + i % RW;
+        s_hash[i] = hash[g];
+        if (spheres) s_sph[i] = spheres[g];
+    }
     __syncthreads();
     for (int r = tid; r < RH; r += nthr) {
         int lo = 0, hi = 0;
@@ -734,6 +815,81 @@ __device__ __forceinline__ int stage_tile(const int2* __restrict__ hash, const i
     }
     __syncthreads();
     return total;
+}
+
+// One tile of one view: stage its halo region and sample its rays.  Output
+// row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
+// persistent kernels can call it in a loop.
+template <int MODE, int KMAX, int SMAX>
+__device__ __forceinline__ void sample_tile_body(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
+                                                 const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
+                                                 const TileSmem& sm) {
+    const int u0 = (tile % tiles_x) * tp.tw;
+    const int v0 = (tile / tiles_x) * tp.th;
+    int ra, va, RW;
+    const int staged = stage_tile(view.hash, view.sorted_idx, o.xyz, o.cull ? view.spheres : nullptr, view.W,
+                                  view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm.pts, sm.hash, sm.sph,
+                                  sm.rowlo, sm.rowbase, sm.total, ra, va, RW);
+    const int u = u0 + (int)threadIdx.x % tp.tw;
+    const int v = v0 + (int)threadIdx.x / tp.tw;
+    if (staged >= 0 && u < view.W && v < view.H) {
+        Ray ray;
+        view_ray(view, u, v, ray);
+        const SrcLds src{sm.hash, sm.pts, sm.sph, ra, va, RW};
+        run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
+    }
+    __syncthreads();  // the LDS is reused by the block's next tile
+}
+
+// A tile whose halo region exceeded the LDS capacity, from global memory.
+template <int MODE, int KMAX, int SMAX>
+__device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o, const SampleOut& out,
+                                                          const ViewArgs& view, const TilePlan& tp, int tile,
+                                                          int tiles_x, int64_t ray_off, int* s_sum) {
+    const int u0 = (tile % tiles_x) * tp.tw;
+    const int v0 = (tile / tiles_x) * tp.th;
+    const int total = region_points(view.hash, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, s_sum);
+    const int u = u0 + (int)threadIdx.x % tp.tw;
+    const int v = v0 + (int)threadIdx.x / tp.tw;
+    if (total > tp.cap && u < view.W && v < view.H) {
+        Ray ray;
+        view_ray(view, u, v, ray);
+        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.spheres, view.W};
+        run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
+    }
+    __syncthreads();
+}
+
+// Bounding sphere of every pixel's points: (centre, R + largest radius), R
+// rounded up so that |p - centre| <= R for each point p; w = -1 for empty
+// pixels.  One thread per pixel.
+__device__ __forceinline__ void bucket_bounds_body(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
+                                                   const float* __restrict__ xyz, const float* __restrict__ radius,
+                                                   float r0, int64_t p, float4* __restrict__ out) {
+    const int2 e = hash[p];
+    if (e.y <= 0) {
+        out[p] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        return;
+    }
+    float mn0 = __builtin_inff(), mn1 = __builtin_inff(), mn2 = __builtin_inff();
+    float mx0 = -__builtin_inff(), mx1 = -__builtin_inff(), mx2 = -__builtin_inff();
+    float rb = r0;
+    for (int j = 0; j < e.y; ++j) {
+        const int id = sorted_idx[e.x + j];
+        const float* q = xyz + 3 * (int64_t)id;
+        mn0 = fminf(mn0, q[0]);
+        mn1 = fminf(mn1, q[1]);
+        mn2 = fminf(mn2, q[2]);
+        mx0 = fmaxf(mx0, q[0]);
+        mx1 = fmaxf(mx1, q[1]);
+        mx2 = fmaxf(mx2, q[2]);
+        if (radius) rb = j == 0 ? radius[id] : fmaxf(rb, radius[id]);
+    }
+    const float c0 = (mn0 + mx0) * 0.5f, c1 = (mn1 + mx1) * 0.5f, c2 = (mn2 + mx2) * 0.5f;
+    const float h0 = (mx0 - mn0) * 0.5f, h1 = (mx1 - mn1) * 0.5f, h2 = (mx2 - mn2) * 0.5f;
+    const float R = sqrtf(sqnorm3(h0, h1, h2)) * (1.0f + 1e-6f) +
+                    1e-6f * (1.0f + fabsf(c0) + fabsf(c1) + fabsf(c2));
+    out[p] = make_float4(c0, c1, c2, R + rb);
 }
 
 }  // namespace epcq

@@ -83,11 +83,13 @@ ABI_SYMBOLS = {
     "epcq_build_hashtable_workspace_size": (_SZ, [_I64, _I32, _I32]),
     "epcq_build_hashtable": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "epcq_quick_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
-                                   _I32, _P, _P, _P, _P, _P]),
+                                   _I32, _P, _P, _P, _P, _P, _SZ, _P]),
+    "epcq_sampling_workspace_size": (_SZ, [_I32, _I32]),
     "epcq_quick_query": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
                                 _I64, _P, _P]),
     "epcq_knn_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
-                                 _P, _I64, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P]),
+                                 _P, _I64, _I32, _I32, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _SZ,
+                                 _P]),
     "epcq_max_live_samples": (_I32, [ctypes.POINTER(_Options), _F32]),
     "epcq_idw_features": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
     "epcq_debug_expf": (_I32, [_P, _P, _I64, _P]),
@@ -359,7 +361,17 @@ def build_hashtable_workspace_size(n, height, width):
     return int(_lib.epcq_build_hashtable_workspace_size(int(n), int(height), int(width)))
 
 
-def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, mode="compat"):
+def _workspace(H, W, device, cull):
+    if not cull:
+        return None
+    return torch.empty(int(_lib.epcq_sampling_workspace_size(H, W)), dtype=torch.uint8, device=device)
+
+
+def _nbytes(t):
+    return 0 if t is None else t.numel() * t.element_size()
+
+
+def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, mode="compat", cull=True):
     """quick_sampling (epcq/csrc/epcq_render_topk.cu:341-417).
 
     Returns (output_sp_xyz float32 [H,W,S,3], output_pc_idx int32 [H,W,S,Kc],
@@ -376,11 +388,13 @@ def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable
     sp_t = torch.empty((H, W, S), dtype=torch.float32, device=device)
     pc_idx = torch.empty((H, W, S, Kc), dtype=torch.int32, device=device)
     mask = torch.empty((H, W, S), dtype=torch.bool, device=device)
+    ws = _workspace(H, W, device, cull)
     camera, opts = _camera_struct(cam), _options_struct(options)
     with torch.cuda.device(device):
         _raise_on(_lib.epcq_quick_sampling(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
                                            _ptr(sorted_point_idx_list), _ptr(hashtable), _MODES[mode],
-                                           _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _stream(device)))
+                                           _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _ptr(ws),
+                                           _nbytes(ws), _stream(device)))
     return sp_xyz, pc_idx, sp_t, mask
 
 
@@ -431,7 +445,7 @@ def max_live_samples(options, r_max):
 
 
 def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, rays=None, radius=None,
-                 mode="knn", s_max=None, feat=None):
+                 mode="knn", s_max=None, feat=None, cull=True):
     """Compact K-nearest sampling.
 
     Same samples as quick_sampling, stored per ray.  Returns a dict with
@@ -487,13 +501,15 @@ def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, 
         bf16 = int(feat.dtype == torch.bfloat16)
         out["w"] = torch.empty((R, s_max, Kc), dtype=torch.float32, device=device)
         out["agg"] = torch.empty((R, s_max, F), dtype=torch.float32, device=device)
+    ws = _workspace(H, W, device, cull)
     camera, opts = _camera_struct(cam), _options_struct(options)
     with torch.cuda.device(device):
         _raise_on(_lib.epcq_knn_sampling(
             _ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts), _ptr(sorted_point_idx_list),
             _ptr(hashtable), _ptr(radius), _ptr(rays), R, _MODES[mode], int(s_max), _ptr(out["n_samples"]),
             _ptr(out["sample_t"]), _ptr(out["nb_idx"]), _ptr(out.get("nb_t")), _ptr(out.get("nb_dist")),
-            _ptr(feat), bf16, F, _ptr(out.get("w")), _ptr(out.get("agg")), _stream(device)))
+            _ptr(feat), bf16, F, _ptr(out.get("w")), _ptr(out.get("agg")), _ptr(ws), _nbytes(ws),
+            _stream(device)))
     return out
 
 

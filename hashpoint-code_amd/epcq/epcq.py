@@ -194,19 +194,19 @@ class NeuralPoints(nn.Module):
                                     sorted_depth_list: torch.Tensor,
                                     hashtable: torch.Tensor,
                                     save_variables_for_debug: bool = False,
-                                    mode: str = "compat"):
+                                    mode: str = "compat", cull: bool = True):
         """(output_sp_xyz, output_pc_idx, output_sp_t, mask_out) of every
         pixel; `save_variables_for_debug` is accepted and ignored."""
         return self.quick_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list,
-                                   hashtable, mode=mode)
+                                   hashtable, mode=mode, cull=cull)
 
     def knn_sampling(self, cam: CameraSpec, opt: RenderOptions, sorted_point_idx_list: torch.Tensor,
                      hashtable: torch.Tensor, rays: Optional[torch.Tensor] = None,
                      radius: Optional[torch.Tensor] = None, mode: str = "knn", feat: Optional[torch.Tensor] = None,
-                     s_max: Optional[int] = None):
+                     s_max: Optional[int] = None, cull: bool = True):
         """Compact K-nearest sampling (see epcq.csrc.knn_sampling)."""
         return _C.knn_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list, hashtable,
-                               rays=rays, radius=radius, mode=mode, s_max=s_max, feat=feat)
+                               rays=rays, radius=radius, mode=mode, s_max=s_max, feat=feat, cull=cull)
 
 
 class FeatureQuery(nn.Module):

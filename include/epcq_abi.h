@@ -140,7 +140,12 @@ int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2c, const fl
 int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
                         const epcq_options* opt, const int32_t* sorted_idx, const int32_t* hash,
                         int32_t mode, float* sp_xyz, int32_t* pc_idx, float* sp_t, uint8_t* mask,
-                        void* stream);
+                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace of the samplers: per-pixel bucket bounds that let the walks skip
+ * neighbour pixels provably outside the radius.  Optional: with a NULL or
+ * smaller workspace the samplers skip nothing (same results). */
+size_t epcq_sampling_workspace_size(int32_t height, int32_t width);
 
 /*
  * quick_query_for_nearby_point -- replaces `quick_query_for_nearby_point`
@@ -173,7 +178,8 @@ int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
                       const float* radius, const int32_t* rays, int64_t R, int32_t mode,
                       int32_t s_max, int32_t* n_samples, float* sample_t, int32_t* nb_idx,
                       float* nb_t, float* nb_dist, const void* feat, int32_t feat_bf16,
-                      int32_t F, float* w, float* agg, void* stream);
+                      int32_t F, float* w, float* agg, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 /* Upper bound on live samples per ray for these options and radius bound. */
 int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);

@@ -42,9 +42,10 @@ def assert_fp_equal(g, o):
         raise AssertionError("fp32 outputs within tolerance but not bit-identical")
 
 
-def check_dense(v, mode="compat"):
+def check_dense(v, mode="compat", cull=True):
     H, W = v.cam.height, v.cam.width
-    sp_xyz, pc_idx, sp_t, mask = v.npts.render_image_hashtable_topk(v.cs, v.opt, v.idx_d, None, v.hash_d, mode=mode)
+    sp_xyz, pc_idx, sp_t, mask = v.npts.render_image_hashtable_topk(v.cs, v.opt, v.idx_d, None, v.hash_d, mode=mode,
+                                                                    cull=cull)
     torch.cuda.synchronize()
     o_xyz, o_idx, o_t, o_mask = oracle.quick_sampling(v.pts, v.ocam, v.oopt, v.oi, v.oh,
                                                       mode=oracle.MODE_KNN if mode == "knn" else oracle.MODE_COMPAT)
@@ -89,7 +90,22 @@ def test_quick_sampling_options(dev, kw, mode):
     pts = synth.sphere_cloud(40_000, seed=5)
     cam = synth.look_at_camera([0.3, -3.5, 0.4], 90, 70, 110.0)  # the sphere touches the borders
     print(f"quick_sampling {mode} {kw}", flush=True)
-    check_dense(View(pts, cam, dev, **kw), mode)
+    v = View(pts, cam, dev, **kw)
+    check_dense(v, mode)
+    check_dense(v, mode, cull=False)
+
+
+@pytest.mark.parametrize("mode", ["compat", "knn"])
+def test_dense_small_view_culling(dev, mode):
+    """A 200x150 view of the 1M-point scene (~30 points per pixel): the
+    bucket culling skips most neighbour pixels; results stay exact."""
+    pts = synth.box_scene(1_000_000, seed=0)
+    cam = synth.look_at_camera([-2.1, 2.9, 1.7], 200, 150, synth.nerf_focal(200))
+    print(f"dense small view {mode}", flush=True)
+    v = View(pts, cam, dev, radius_threshold=0.005, num_point_cloud_per_sp=8)
+    live = check_dense(v, mode)
+    assert live.sum() > 5_000
+    compact_vs_dense(v, mode)
 
 
 def test_quick_sampling_box_knn(dev):
@@ -98,10 +114,10 @@ def test_quick_sampling_box_knn(dev):
     check_dense(View(pts, cam, dev, radius_threshold=0.01, num_point_cloud_per_sp=8), "knn")
 
 
-def compact_vs_dense(v, mode, rays=None, feat=None, radius=None):
+def compact_vs_dense(v, mode, rays=None, feat=None, radius=None, cull=True):
     H, W = v.cam.height, v.cam.width
     out = v.npts.knn_sampling(v.cs, v.opt, v.idx_d, v.hash_d, rays=to_dev(rays, v.dev) if rays is not None else None,
-                              radius=v.radius_d, mode=mode, feat=feat)
+                              radius=v.radius_d, mode=mode, feat=feat, cull=cull)
     torch.cuda.synchronize()
     res = oracle.quick_sampling(v.pts, v.ocam, v.oopt, v.oi, v.oh, radius=v.radius,
                                 mode=oracle.MODE_KNN if mode == "knn" else oracle.MODE_COMPAT, rays=rays,
@@ -160,6 +176,7 @@ def test_knn_sampling_per_point_radius_k16(dev):
     feat = torch.randn(len(pts), 32, generator=torch.Generator().manual_seed(2)).to(torch.bfloat16).to(dev)
     print("knn_sampling per-point radius K=16", flush=True)
     compact_vs_dense(v, "knn", feat=feat)
+    compact_vs_dense(v, "knn", cull=False)
 
 
 def test_quick_query_vs_oracle(dev):
