@@ -228,7 +228,7 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
 
 ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash, const float4* spheres) {
     return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
-                    reinterpret_cast<const int2*>(hash), sorted_idx, spheres};
+                    reinterpret_cast<const int2*>(hash), sorted_idx, spheres, nullptr};
 }
 
 // Bucket bounds into the workspace (when it is large enough); returns the

@@ -677,6 +677,7 @@ struct ViewArgs {
     const int2* hash;
     const int* sorted_idx;
     const float4* spheres;  // bucket bounds or nullptr
+    const float* xyz;       // the view's cloud (nullptr: SampleOpts::xyz)
 };
 
 __device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
@@ -821,9 +822,11 @@ __device__ __forceinline__ int stage_tile(const int2* __restrict__ hash, const i
 // row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
 // persistent kernels can call it in a loop.
 template <int MODE, int KMAX, int SMAX>
-__device__ __forceinline__ void sample_tile_body(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
+__device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                  const TileSmem& sm) {
+    SampleOpts o = o0;
+    if (view.xyz) o.xyz = view.xyz;
     const int u0 = (tile % tiles_x) * tp.tw;
     const int v0 = (tile / tiles_x) * tp.th;
     int ra, va, RW;
@@ -843,9 +846,11 @@ __device__ __forceinline__ void sample_tile_body(const SampleOpts& o, const Samp
 
 // A tile whose halo region exceeded the LDS capacity, from global memory.
 template <int MODE, int KMAX, int SMAX>
-__device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o, const SampleOut& out,
+__device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, const SampleOut& out,
                                                           const ViewArgs& view, const TilePlan& tp, int tile,
                                                           int tiles_x, int64_t ray_off, int* s_sum) {
+    SampleOpts o = o0;
+    if (view.xyz) o.xyz = view.xyz;
     const int u0 = (tile % tiles_x) * tp.tw;
     const int v0 = (tile / tiles_x) * tp.th;
     const int total = region_points(view.hash, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, s_sum);

@@ -65,6 +65,19 @@ TUNE_XCD_ORDER = 1
 TUNE_PREFILTER = 2
 
 
+class _ViewDesc(ctypes.Structure):
+    _fields_ = [("xyz", ctypes.c_void_p), ("n", ctypes.c_int64), ("c2w", ctypes.c_float * 12),
+                ("w2c", ctypes.c_float * 12), ("K", ctypes.c_float * 9), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32)]
+
+
+class _BatchViewOut(ctypes.Structure):
+    _fields_ = [("sorted_idx", ctypes.c_void_p), ("hash", ctypes.c_void_p), ("rays", ctypes.c_int64),
+                ("ray_offset", ctypes.c_int64), ("n_samples", ctypes.c_void_p), ("sample_t", ctypes.c_void_p),
+                ("nb_idx", ctypes.c_void_p), ("nb_t", ctypes.c_void_p), ("nb_dist", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -93,6 +106,17 @@ ABI_SYMBOLS = {
     "epcq_max_live_samples": (_I32, [ctypes.POINTER(_Options), _F32]),
     "epcq_idw_features": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
     "epcq_debug_expf": (_I32, [_P, _P, _I64, _P]),
+    "epcq_batch_arena_bytes": (_SZ, [ctypes.POINTER(_ViewDesc), _I32, _I32, ctypes.POINTER(_Options), _I32, _I32]),
+    "epcq_batch_create": (_I32, [ctypes.POINTER(_Options), _I32, _I32, _I32, _P, _F32, ctypes.POINTER(_Tuning), _P,
+                                 _SZ, ctypes.POINTER(ctypes.c_void_p)]),
+    "epcq_batch_set_views": (_I32, [_P, ctypes.POINTER(_ViewDesc), _I32, _P]),
+    "epcq_batch_view_outputs": (_I32, [_P, _I32, ctypes.POINTER(_BatchViewOut)]),
+    "epcq_batch_rays": (_I64, [_P]),
+    "epcq_batch_run": (_I32, [_P, _P]),
+    "epcq_batch_capture": (_I32, [_P, _P]),
+    "epcq_batch_launch": (_I32, [_P, _P]),
+    "epcq_batch_destroy": (None, [_P]),
+    "epcq_batch_profile": (_I32, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
 }
 
 

@@ -193,6 +193,60 @@ int epcq_idw_features(const float* xyz, const void* feat, int32_t feat_bf16, int
                       const float* sp_xyz, const int32_t* nb_idx, int64_t M, int32_t K,
                       float* w, float* agg, void* stream);
 
+/*
+ * Batched multi-view sampling (no counterpart in the reference, which renders
+ * one view per call).  A batch holds many views -- each its own camera,
+ * resolution and point cloud -- and runs the hash-table build, bucket bounds
+ * and compact sampling of all of them in one persistent pass, replayable as
+ * a HIP graph.  All device memory is the caller's arena.
+ */
+typedef struct epcq_view_desc {
+    const float* xyz;  /* device [n, 3] point cloud of this view */
+    int64_t n;
+    float c2w[12];     /* row-major 3 x 4 */
+    float w2c[12];     /* row-major 3 x 4 */
+    float K[9];        /* row-major 3 x 3 */
+    float fx, fy, cx, cy;
+    int32_t width, height;
+} epcq_view_desc;
+
+typedef struct epcq_batch epcq_batch; /* opaque host object */
+
+/* Device pointers of one view's results inside the arena (layout as
+ * epcq_knn_sampling's compact outputs; nb_t / nb_dist NULL in compat mode). */
+typedef struct epcq_batch_view_out {
+    int32_t* sorted_idx;  /* [n] pixel-ordered point lists */
+    int32_t* hash;        /* [H*W, 2] */
+    int64_t rays;         /* H*W */
+    int64_t ray_offset;   /* first row of this view in the batch outputs */
+    int32_t* n_samples;   /* [H*W] */
+    float* sample_t;      /* [H*W, s_max] */
+    int32_t* nb_idx;      /* [H*W, s_max, Kc] */
+    float* nb_t;
+    float* nb_dist;
+} epcq_batch_view_out;
+
+/* Arena bytes for these views (capacity for up to max_views descriptors). */
+size_t epcq_batch_arena_bytes(const epcq_view_desc* views, int32_t n_views, int32_t max_views,
+                              const epcq_options* opt, int32_t mode, int32_t s_max);
+/* radius: optional per-point radius of the (single) cloud, r_max its maximum;
+ * tuning: launch shape (NULL = the process-wide setting). */
+int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t s_max, int32_t max_views,
+                      const float* radius, float r_max, const epcq_tuning* tuning, void* arena,
+                      size_t arena_bytes, epcq_batch** out);
+/* Lays the views out in the arena and uploads their descriptors (returns
+ * after the upload).  A captured graph stays valid for any view set. */
+int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, int32_t n_views, void* stream);
+int epcq_batch_view_outputs(const epcq_batch* b, int32_t view, epcq_batch_view_out* out);
+int64_t epcq_batch_rays(const epcq_batch* b);
+int epcq_batch_run(epcq_batch* b, void* stream);      /* enqueue the pass */
+int epcq_batch_capture(epcq_batch* b, void* stream);  /* capture the pass into a HIP graph */
+int epcq_batch_launch(epcq_batch* b, void* stream);   /* replay the graph */
+/* Runs the pass eagerly with HIP events between its stages and waits for it:
+ * stage_ms[0..2] = build, bucket bounds, sampling (milliseconds). */
+int epcq_batch_profile(epcq_batch* b, void* stream, float* stage_ms);
+void epcq_batch_destroy(epcq_batch* b);
+
 /* Test hook: our exp() on the device, out[i] = exp(in[i]). */
 int epcq_debug_expf(const float* in, float* out, int64_t n, void* stream);
 
