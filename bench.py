@@ -1,12 +1,24 @@
 """Benchmark of the epcq searching + sampling path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload box|sphere|batched|large]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload batched|box|sphere|large] [--mode compat|knn]
 
-A step is one pass of the path over the workload's views: per view the hash
-table build (fused pts2plane) and the K-nearest sampling (samples, neighbour
-indices, ray depths, point-to-ray distances and, where the workload has
-features, inverse-distance weights and weighted features).  Inputs are
-resident in HBM before timing starts.  Prints one JSON line (rank 0).
+Default workload (batched): the 1M-point 20-box scene seen by 64 views of
+mixed resolution ({200x150, 400x300, 640x512, 800x800, 1600x1200}, f = 1111 *
+W / 800, eyes on a sphere of radius 4), r = 0.005, K = 8, fp32.  A step is one
+pass over all 64 views: per view the hash-table build (projection, per-pixel
+point lists, hash table) and the sampling of every pixel's ray (samples,
+neighbour indices and, in knn mode, neighbour depths and point-to-ray
+distances).  The batched form runs a step as one replay of a captured HIP
+graph of persistent kernels with an autotuned launch shape; `value` is its
+rays/s.  Beside it the same 64 views go through the plain per-view drop-in
+calls (pts2plane + render_image_hashtable_topk, view after view, as the
+reference's renderer does), and through per-view compact sampling; the
+ratios are reported.  Inputs are resident in HBM before timing starts.
+
+Workloads box / sphere / large time the per-view path on one or sixteen
+views (the single-view scene configurations and the 20M-point one).
+Prints one JSON line.
 """
 import argparse
 import json
@@ -18,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hashpoint-code_amd"))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "rays/s search+sample, batched/served vs plain per-view; p50/p99 latency"
 
 
 def parse():
@@ -25,11 +38,18 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="box", choices=["box", "sphere", "batched", "large"])
-    ap.add_argument("--mode", default="knn", choices=["knn", "compat"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "serving", "box", "sphere", "large"])
+    ap.add_argument("--mode", default="compat", choices=["compat", "knn"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-tune", action="store_true", help="batched: default launch shape, no autotuning")
+    ap.add_argument("--no-serve", action="store_true", help="batched: skip the serving sweep")
+    ap.add_argument("--plain-passes", type=int, default=3, help="batched: timed passes of the per-view forms")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def load_workload(name):
@@ -47,21 +67,292 @@ def load_workload(name):
     return dict(name="box200-20M-16x1920x1080", pts=pts, cams=cams, kw=kw, radius=radius, F=32, feat_dtype="bf16")
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(views would be sharded round-robin across GPUs, see the design notes)", file=sys.stderr)
-        sys.exit(2)
+def camera_specs(cams, dev):
+    import numpy as np
+    import torch
+    from epcq.epcq import CameraSpec
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    return [CameraSpec(c2w=t(c.c2w), w2c=t(c.w2c), K=t(c.K), fx=c.fx, fy=c.fy, cx=c.cx, cy=c.cy,
+                       width=c.width, height=c.height) for c in cams]
 
+
+def percentiles(ms):
+    import numpy as np
+    a = np.asarray(ms, dtype=np.float64)
+    return {"p50_ms": float(np.percentile(a, 50)), "p99_ms": float(np.percentile(a, 99)), "mean_ms": float(a.mean())}
+
+
+def out_bytes_per_ray(s_max, K, knn, F=0, fe=4):
+    """Bytes the sampler writes per ray: count, s_max depths, s_max x K
+    indices (+ depths and distances in knn mode, + weights and aggregated
+    features with features)."""
+    b = 4 + s_max * (4 + 4 * K)
+    if knn:
+        b += s_max * 8 * K
+    if F:
+        b += s_max * (4 * K + 4 * F)
+    return b
+
+
+# ----------------------------------------------------------------------------
+def bench_batched(args, wl):
     import numpy as np
     import torch
 
     import epcq.csrc as C
-    from epcq.epcq import CameraSpec, NeuralPoints, RenderOptions
+    from epcq import autotune
+    from epcq.batch import MultiViewSampler
+    from epcq.epcq import NeuralPoints, RenderOptions
 
     dev = torch.device("cuda:0")
-    wl = load_workload(args.workload)
+    opt = RenderOptions(**wl["kw"])
+    xyz = torch.from_numpy(wl["pts"]).to(dev)
+    n = xyz.shape[0]
+    cams = wl["cams"]
+    pairs = [(xyz, c) for c in cams]
+    V = len(pairs)
+    rays_per_step = sum(c.width * c.height for c in cams)
+    K = opt.num_point_cloud_per_sp
+    knn = args.mode == "knn"
+
+    # ---- autotuned launch shape for the batch ---------------------------------
+    probe = MultiViewSampler(opt, mode=args.mode, device=dev, max_views=V)
+    nbytes = probe.required_bytes(pairs)
+    s_max = probe.s_max
+    probe.close()
+    del probe
+    arena = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    tune_log = {}
+    if args.no_tune:
+        best = dict(autotune.DEFAULT)
+    else:
+        cands = {}
+
+        def run(t):
+            key = (t["tile_w"], t["tile_h"], t["tile_cap"], t["flags"])
+            if key not in cands:
+                torch.cuda.synchronize()  # the previous candidate is idle before its arena is rewritten
+                s = MultiViewSampler(opt, mode=args.mode, device=dev, max_views=V, tuning=t)
+                s.adopt_arena(arena)
+                s.set_views(pairs)
+                cands.clear()  # one live candidate at a time (they share the arena)
+                cands[key] = s
+            cands[key].run()
+
+        t0 = time.perf_counter()
+        best, tune_log = autotune.tune(f"batch/{wl['name']}/{args.mode}", run, reps=2, log=log)
+        for s in cands.values():
+            s.close()
+        cands.clear()
+        log(f"autotune: best {best} in {time.perf_counter() - t0:.1f} s")
+
+    sampler = MultiViewSampler(opt, mode=args.mode, device=dev, max_views=V, tuning=best)
+    sampler.adopt_arena(arena)
+    sampler.set_views(pairs)
+    assert sampler.rays == rays_per_step
+    sampler.capture()
+    stream = sampler.stream
+
+    # ---- batched: graph replays -----------------------------------------------
+    for _ in range(args.warmup):
+        sampler.replay()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        sampler.replay()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [a.elapsed_time(b) for a, b in evs]
+    value = rays_per_step * args.steps / wall
+    log(f"batched: {wall * 1e3 / args.steps:.3f} ms/step, {value / 1e9:.3f} G rays/s")
+
+    # stage split of the same pass (eager, events between stages on its stream)
+    prof = [sampler.profile() for _ in range(max(3, min(args.steps, 10)))]
+    torch.cuda.synchronize()
+    stage = {k: float(np.median([p[k] for p in prof])) for k in ("build", "bounds", "sample")}
+    n_in = 0
+    n_samples = 0
+    for i in range(V):
+        o = sampler.view_outputs(i)
+        n_in += int(o["hash"][:, 1].sum().item())
+        n_samples += int(o["n_samples"].sum().item())
+    build_bytes = V * 16 * n + 8 * rays_per_step
+    sample_bytes = 8 * rays_per_step + 16 * n_in + out_bytes_per_ray(s_max, K, knn) * rays_per_step
+    sample_gbs = sample_bytes / (stage["sample"] * 1e-3) / 1e9
+    build_gbs = build_bytes / (stage["build"] * 1e-3) / 1e9
+
+    # ---- plain per-view drop-in and per-view compact forms ---------------------
+    npts = NeuralPoints(pc_type="pc", points=xyz)
+    specs = camera_specs(cams, dev)
+    ws = torch.empty(max(C.build_hashtable_workspace_size(n, c.height, c.width) for c in cams),
+                     dtype=torch.uint8, device=dev)
+
+    def plain_pass():
+        for cs in specs:
+            idx, depth, table = npts.pts2plane(cs, opt, workspace=ws)
+            out = npts.render_image_hashtable_topk(cs, opt, idx, depth, table, mode=args.mode)
+            del out, idx, depth, table
+
+    def compact_pass():
+        for cs in specs:
+            idx, depth, table = npts.pts2plane(cs, opt, workspace=ws)
+            out = npts.knn_sampling(cs, opt, idx, table, mode=args.mode, s_max=s_max)
+            del out, idx, depth, table
+
+    def timed(fn, passes):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(passes):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / passes
+
+    C.set_tuning(**best)
+    plain_s = timed(plain_pass, args.plain_passes)
+    compact_s = timed(compact_pass, args.plain_passes)
+    plain = rays_per_step / plain_s
+    compact = rays_per_step / compact_s
+    log(f"plain per-view drop-in: {plain_s * 1e3:.2f} ms/pass; per-view compact: {compact_s * 1e3:.2f} ms/pass")
+
+    record = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "points": n, "views": V, "rays_per_step": rays_per_step,
+                   "resolutions": "200x150,400x300,640x512,800x800,1600x1200 (uniform draw)",
+                   "radius": float(opt.radius_threshold), "K": K, "mode": args.mode, "s_max": s_max,
+                   "form": "batched: one captured HIP graph per step", "tuning": best},
+        "latency_per_step": percentiles(step_ms),
+        "plain_per_view": {"value": plain, "unit": "rays/s", "ms_per_pass": plain_s * 1e3,
+                           "calls": "pts2plane + render_image_hashtable_topk (dense outputs) per view",
+                           "batched_speedup": value / plain},
+        "per_view_compact": {"value": compact, "unit": "rays/s", "ms_per_pass": compact_s * 1e3,
+                             "calls": "pts2plane + knn_sampling (compact outputs) per view",
+                             "batched_speedup": value / compact},
+        "stages_ms": stage,
+        "samples_per_step": n_samples,
+        "roofline": {"bound": "hbm", "kernel": "batch_sample_kernel + batch_fallback_kernel",
+                     "achieved": sample_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sample_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "avg_ms": stage["sample"], "bytes_per_launch": sample_bytes,
+                     "bytes_model": "8 B/ray hash + 16 B per in-view point + written outputs per ray"},
+        "build": {"avg_ms": stage["build"], "achieved_GBs": build_gbs, "frac": build_gbs / HBM_PEAK_GBS,
+                  "bytes_per_step": build_bytes, "bytes_model": "16 B/point/view + 8 B/pixel"},
+    }
+    if tune_log:
+        record["autotune_ms"] = tune_log
+    sampler.close()
+    del sampler, arena
+    if not args.no_serve:
+        record["serving"] = bench_serving(args, tuning=best)
+    if not args.no_cpu:
+        record["cpu_baseline"] = cpu_baseline_views(wl, opt, args.cpu_seconds, args.mode)
+    return record
+
+
+# ----------------------------------------------------------------------------
+def serving_clouds(dev):
+    """The four resident clouds: 0.5M sphere, 1M / 20 boxes, 2M / 40 boxes,
+    5M / 80 boxes (each box scene with its ground plane)."""
+    import torch
+    from epcq import synth
+    t = lambda a: torch.from_numpy(a).to(dev)
+    return {
+        "sphere-0.5M": t(synth.sphere_cloud(500_000, seed=10)),
+        "box20-1M": t(synth.box_scene(1_000_000, n_boxes=20, seed=11)),
+        "box40-2M": t(synth.box_scene(2_000_000, n_boxes=40, seed=12)),
+        "box80-5M": t(synth.box_scene(5_000_000, n_boxes=80, seed=13)),
+    }
+
+
+def bench_serving(args, fractions=(0.3, 0.6, 0.9), seconds=4.0, tuning=None, clouds=None):
+    """Poisson request streams against the continuous-batching server.  The
+    capacity (requests/s with every request queued at t = 0) sets the swept
+    rates: `fractions` of it, each for about `seconds` of arrivals (at least
+    200 requests).  Latency = completion seen by the host - scheduled
+    arrival."""
+    import numpy as np
+    import torch
+
+    from epcq.epcq import RenderOptions
+    from epcq.serve import PointServer, poisson_trace, run_trace
+
+    dev = torch.device("cuda:0")
+    clouds = clouds or serving_clouds(dev)
+    opt = RenderOptions(radius_threshold=0.005, num_point_cloud_per_sp=8, topK=8)
+    server = PointServer(clouds, opt, mode=args.mode, slots=4, slot_bytes=3 << 30, max_views=32, tuning=tuning)
+
+    def summarize(reqs, wall):
+        lat = np.array([q.latency for q in reqs]) * 1e3
+        rays = sum(q.rays for q in reqs)
+        return {"requests": len(reqs), "rays_per_s": rays / wall, "req_per_s": len(reqs) / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean())}
+
+    # warm: capture every slot's graph
+    warm = poisson_trace(1e9, 4 * len(server.slots), clouds, seed=19)
+    run_trace(server, warm)
+    # capacity: everything queued at once
+    burst = poisson_trace(1e9, 400, clouds, seed=21)
+    burst = [(0.0, c, cam) for _, c, cam in burst]
+    t0 = time.perf_counter()
+    reqs = run_trace(server, burst)
+    wall = time.perf_counter() - t0
+    cap = summarize(reqs, wall)
+    log(f"serving capacity: {cap['req_per_s']:.0f} req/s, {cap['rays_per_s'] / 1e9:.3f} G rays/s")
+    rates = []
+    for i, f in enumerate(fractions):
+        rate = f * cap["req_per_s"]
+        count = int(max(200, min(4000, rate * seconds)))
+        trace = poisson_trace(rate, count, clouds, seed=20 + 10 * i)
+        t0 = time.perf_counter()
+        reqs = run_trace(server, trace)
+        wall = time.perf_counter() - t0
+        r = summarize(reqs, wall)
+        r["rate_req_per_s"] = rate
+        rates.append(r)
+        log(f"serving at {rate:.0f} req/s: p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, "
+            f"{r['rays_per_s'] / 1e9:.3f} G rays/s")
+    out = {"clouds": {k: int(v.shape[0]) for k, v in clouds.items()}, "slots": len(server.slots),
+           "slot_bytes": server.slots[0].nbytes, "max_views_per_batch": server.max_views,
+           "max_inflight_slots": server.max_inflight, "capacity": cap, "rates": rates,
+           "resolutions": "200x150,400x300,640x512,800x800,1600x1200 (uniform draw)", "mode": args.mode}
+    server.close()
+    return out
+
+
+def serving_record(args):
+    s = bench_serving(args, fractions=(0.2, 0.4, 0.6, 0.8, 0.95), seconds=6.0)
+    top = max(s["rates"], key=lambda r: r["rate_req_per_s"])
+    return {"metric": METRIC, "value": top["rays_per_s"], "unit": "rays/s", "n_gpus": 1,
+            "steps": top["requests"], "warmup": 4 * s["slots"], "ms_per_step": top["mean_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "serving-4clouds-poisson", "form": "served, continuous batching",
+                       "radius": 0.005, "K": 8, "mode": args.mode},
+            "serving": s}
+
+
+# ----------------------------------------------------------------------------
+def bench_per_view(args, wl):
+    import numpy as np
+    import torch
+
+    import epcq.csrc as C
+    from epcq.epcq import NeuralPoints, RenderOptions
+
+    dev = torch.device("cuda:0")
     pts = wl["pts"]
     n = len(pts)
     opt = RenderOptions(**wl["kw"])
@@ -74,34 +365,27 @@ def main():
         if wl["feat_dtype"] == "bf16":
             feat = feat.to(torch.bfloat16)
     npts = NeuralPoints(pc_type="pc", points=xyz)
-    views = []
-    for c in wl["cams"]:
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-        views.append(CameraSpec(c2w=t(c.c2w), w2c=t(c.w2c), K=t(c.K), fx=c.fx, fy=c.fy, cx=c.cx, cy=c.cy,
-                                width=c.width, height=c.height))
+    views = camera_specs(wl["cams"], dev)
     rays_per_step = sum(c.width * c.height for c in wl["cams"])
     r_max = float(wl["radius"].max()) if wl["radius"] is not None else float(opt.radius_threshold)
     s_max = max(1, C.max_live_samples(opt, r_max))
     ws = torch.empty(max(C.build_hashtable_workspace_size(n, c.height, c.width) for c in wl["cams"]),
                      dtype=torch.uint8, device=dev)
-
     stream = torch.cuda.current_stream(dev)
-    ev = []  # (build_start, build_end, sample_end) per view per timed step
+    ev = []
 
     def step(record):
         for cs in views:
-            e0 = torch.cuda.Event(enable_timing=True) if record else None
-            e1 = torch.cuda.Event(enable_timing=True) if record else None
-            e2 = torch.cuda.Event(enable_timing=True) if record else None
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if record else None
             if record:
-                e0.record(stream)
+                e[0].record(stream)
             idx, depth, table = npts.pts2plane(cs, opt, workspace=ws)
             if record:
-                e1.record(stream)
+                e[1].record(stream)
             out = npts.knn_sampling(cs, opt, idx, table, radius=radius, mode=args.mode, feat=feat, s_max=s_max)
             if record:
-                e2.record(stream)
-                ev.append((e0, e1, e2, cs))
+                e[2].record(stream)
+                ev.append(e)
             del out, idx, depth, table
 
     for _ in range(args.warmup):
@@ -112,94 +396,117 @@ def main():
         step(True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-
-    build_ms = np.array([a.elapsed_time(b) for a, b, _, _ in ev])
-    sample_ms = np.array([b.elapsed_time(c) for _, b, c, _ in ev])
-    ms_per_step = wall * 1e3 / args.steps
+    build_ms = np.array([a.elapsed_time(b) for a, b, _ in ev])
+    sample_ms = np.array([b.elapsed_time(c) for _, b, c in ev])
     value = rays_per_step * args.steps / wall
 
-    # algorithmic bytes per launch (one view), as in the design notes
-    P_total = rays_per_step
-    views_per_step = len(views)
+    V = len(views)
     K = opt.num_point_cloud_per_sp
     F = wl["F"]
     fe = 2 if wl["feat_dtype"] == "bf16" else 4
-    b_out = 4 + s_max * (4 + 12 * K) + ((s_max * (4 * K + 4 * F)) if F else 0)
-    build_bytes = views_per_step * (12 * n + 8 * n) + 8 * P_total
-    # sample kernel reads: hash 8 B/pixel, each in-image point's index + xyz
-    # once (16 B; N_in <= N), features once per point; writes b_out per ray
-    sample_bytes = 8 * P_total + views_per_step * (16 * n + (F * fe * n if F else 0)) + b_out * P_total
-    sample_avg_ms = float(sample_ms.mean())
-    build_avg_ms = float(build_ms.mean())
-    sample_gbs = sample_bytes / views_per_step / (sample_avg_ms * 1e-3) / 1e9
-    build_gbs = build_bytes / views_per_step / (build_avg_ms * 1e-3) / 1e9
-
+    # hash table in use by the last view: in-view points
+    idx, depth, table = npts.pts2plane(views[0], opt, workspace=ws)
+    n_in = int(table.view(-1, 2)[:, 1].sum().item())
+    P0 = views[0].width * views[0].height
+    build_bytes = 20 * n + 8 * P0  # pts2plane: xyz in; sorted idx, depth, hash out
+    sample_bytes = (8 * P0 + n_in * (16 + (4 if radius is not None else 0) + (F * fe if F else 0))
+                    + out_bytes_per_ray(s_max, K, args.mode == "knn", F) * P0)
+    sample_gbs = sample_bytes / (float(sample_ms.mean()) * 1e-3) / 1e9
+    build_gbs = build_bytes / (float(build_ms.mean()) * 1e-3) / 1e9
     record = {
-        "metric": "rays/s search+sample, batched/served vs plain per-view; p50/p99 latency",
+        "metric": METRIC,
         "value": value,
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16 features, f32 geometry" if wl["feat_dtype"] == "bf16" else "f32",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "points": n, "views": views_per_step, "rays_per_step": rays_per_step,
-                   "radius": r_max if wl["radius"] is not None else float(opt.radius_threshold), "K": K,
-                   "features": F, "mode": args.mode, "form": "plain per-view"},
-        "roofline": {"bound": "hbm", "kernel": "sample_kernel (knn_sampling)", "achieved": sample_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sample_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "avg_ms": sample_avg_ms, "bytes_per_launch": sample_bytes / views_per_step},
-        "build": {"avg_ms": build_avg_ms, "achieved_GBs": build_gbs, "frac": build_gbs / HBM_PEAK_GBS,
-                  "bytes_per_view": build_bytes / views_per_step},
+        "config": {"workload": wl["name"], "points": n, "views": V, "rays_per_step": rays_per_step,
+                   "radius": r_max, "per_point_radius": radius is not None, "K": K, "features": F,
+                   "mode": args.mode, "s_max": s_max, "form": "per-view"},
+        "latency_per_view": percentiles(build_ms + sample_ms),
+        "roofline": {"bound": "hbm", "kernel": "sample_tile_kernel + sample_tile_fallback_kernel (knn_sampling)",
+                     "achieved": sample_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sample_gbs / HBM_PEAK_GBS, "traffic": None, "avg_ms": float(sample_ms.mean()),
+                     "bytes_per_launch": sample_bytes},
+        "build": {"avg_ms": float(build_ms.mean()), "achieved_GBs": build_gbs, "frac": build_gbs / HBM_PEAK_GBS,
+                  "bytes_per_view": build_bytes},
     }
     if not args.no_cpu:
-        record["cpu_baseline"] = cpu_baseline(wl, opt, args.cpu_seconds, s_max, args.mode)
-    print(json.dumps(record), flush=True)
+        record["cpu_baseline"] = cpu_baseline_views(wl, opt, args.cpu_seconds, args.mode)
+    return record
 
 
-def cpu_baseline(wl, opt, target_s, s_max, mode):
+# ----------------------------------------------------------------------------
+def cpu_baseline_views(wl, opt, target_s, mode, max_views=4):
     """The oracle (a C++ restatement of the reference path, OpenMP) on the
-    host cores: full hash build of the first view + sampling of a row-block
-    sample of its rays; rays/s = P / (t_build + t_sample * P / sampled)."""
+    host cores, over a bounded sample: the full hash build of up to
+    `max_views` views and the sampling of a central row block of each, grown
+    until the sample takes about `target_s`.  rays/s of the whole workload =
+    total rays / (mean build time x views + sampling time per ray x total
+    rays)."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     oracle.set_threads(threads)
-    cam = wl["cams"][0]
     pts = wl["pts"]
-    P = cam.width * cam.height
-    t0 = time.perf_counter()
-    oi, od, oh = oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width)
-    t_build = time.perf_counter() - t0
-    ocam = oracle.camera_from(cam)
     oopt = oracle.options_from(opt)
     m = oracle.MODE_KNN if mode == "knn" else oracle.MODE_COMPAT
-    # grow the sample until it takes ~target_s
-    rows = max(1, cam.height // 64)
-    sampled, t_sample = 0, 0.0
-    while True:
-        rays = np.arange(0, rows * cam.width, dtype=np.int32) + (cam.height // 2 - rows // 2) * cam.width
-        rays = rays[(rays >= 0) & (rays < P)]
-        t1 = time.perf_counter()
-        res = oracle.quick_sampling(pts, ocam, oopt, oi, oh, radius=wl["radius"], mode=m, rays=rays, with_nb=True)
-        if wl["F"]:
-            live = res[3].reshape(-1)
-            feat = np.zeros((len(pts), wl["F"]), np.float32)
-            oracle.idw(pts, feat, res[0].reshape(-1, 3)[live], res[1].reshape(-1, res[1].shape[-1])[live])
-        t_sample = time.perf_counter() - t1
-        sampled = rays.size
-        if t_sample >= target_s / 4 or rows >= cam.height:
-            break
-        rows = min(cam.height, rows * 4)
-    rate = P / (t_build + t_sample * P / sampled)
+    cams = wl["cams"]
+    sub = cams[:max_views]
+    t_builds, t_samp, n_samp = [], 0.0, 0
+    budget = target_s / len(sub)
+    for cam in sub:
+        t0 = time.perf_counter()
+        oi, od, oh = oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width)
+        t_builds.append(time.perf_counter() - t0)
+        ocam = oracle.camera_from(cam)
+        P = cam.width * cam.height
+        rows = max(1, cam.height // 64)
+        while True:
+            first = max(0, cam.height // 2 - rows // 2) * cam.width
+            rays = np.arange(first, min(P, first + rows * cam.width), dtype=np.int32)
+            t1 = time.perf_counter()
+            res = oracle.quick_sampling(pts, ocam, oopt, oi, oh, radius=wl["radius"], mode=m, rays=rays, with_nb=True)
+            if wl["F"]:
+                live = res[3].reshape(-1)
+                feat = np.zeros((len(pts), wl["F"]), np.float32)
+                oracle.idw(pts, feat, res[0].reshape(-1, 3)[live], res[1].reshape(-1, res[1].shape[-1])[live])
+            dt = time.perf_counter() - t1
+            if dt >= budget / 4 or rows >= cam.height:
+                t_samp += dt
+                n_samp += rays.size
+                break
+            rows = min(cam.height, rows * 4)
+    total = sum(c.width * c.height for c in cams)
+    rate = total / (float(np.mean(t_builds)) * len(cams) + t_samp / n_samp * total)
     return {"value": rate, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"view 0 of the workload: full hash build ({t_build:.3f} s) + sampling of {sampled} "
-                      f"central rays ({t_sample:.3f} s), oracle with {threads} OpenMP threads"}
+            "sample": f"{len(sub)} of {len(cams)} views: full hash build (mean {np.mean(t_builds):.3f} s) + "
+                      f"sampling of {n_samp} central rays ({t_samp:.2f} s), {mode} mode, scaled to "
+                      f"{total} rays; oracle with {threads} OpenMP threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(views would be sharded round-robin across GPUs, one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "serving":
+        print(json.dumps(serving_record(args)), flush=True)
+        return
+    wl = load_workload(args.workload)
+    if args.workload == "batched":
+        record = bench_batched(args, wl)
+    else:
+        record = bench_per_view(args, wl)
+    print(json.dumps(record), flush=True)
 
 
 if __name__ == "__main__":

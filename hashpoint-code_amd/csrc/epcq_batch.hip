@@ -353,6 +353,17 @@ extern "C" size_t epcq_batch_arena_bytes(const epcq_view_desc* views, int32_t n_
     return total;
 }
 
+extern "C" size_t epcq_batch_header_bytes(int32_t max_views) {
+    return max_views < 1 ? 0 : header_bytes(max_views);
+}
+
+extern "C" size_t epcq_batch_view_bytes(const epcq_view_desc* view, const epcq_options* opt, int32_t mode,
+                                        int32_t s_max) {
+    if (!view || !opt) return 0;
+    return layout_view(*view).bytes +
+           output_bytes((int64_t)view->width * view->height, s_max, opt->num_point_cloud_per_sp, mode);
+}
+
 extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t s_max, int32_t max_views,
                                  const float* radius, float r_max, const epcq_tuning* tuning, void* arena,
                                  size_t arena_bytes, epcq_batch** out) {
@@ -392,9 +403,7 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     b->V = 0;
     b->total_rays = 0;
     b->grid_build = device_grid(8);
-    int occ = 0;
     b->grid_sample = device_grid(4);
-    (void)occ;
     b->graph = nullptr;
     b->exec = nullptr;
     *out = b;

@@ -66,11 +66,13 @@ class MultiViewSampler:
         self.views = []
         self.captured = False
 
-    def _create(self, nbytes):
+    def _create(self, nbytes=None, arena=None):
         if self.handle:
             _lib.epcq_batch_destroy(self.handle)
             self.handle = ctypes.c_void_p(0)
-        self.arena = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        if arena is None:
+            arena = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        self.arena = arena
         tu = None
         if self.tuning is not None:
             tu = ctypes.byref(_Tuning(self.tuning["tile_w"], self.tuning["tile_h"], self.tuning["tile_cap"],
@@ -88,10 +90,28 @@ class MultiViewSampler:
         return int(_lib.epcq_batch_arena_bytes(descs, len(pairs), self.max_views, ctypes.byref(self.opts),
                                                _C._MODES[self.mode], self.s_max))
 
+    def header_bytes(self):
+        """Arena bytes of the descriptor header (for max_views views)."""
+        return int(_lib.epcq_batch_header_bytes(self.max_views))
+
+    def view_bytes(self, xyz, cam):
+        """Upper bound of the arena bytes one view adds to a batch."""
+        d = view_desc(xyz, cam)
+        return int(_lib.epcq_batch_view_bytes(ctypes.byref(d), ctypes.byref(self.opts), _C._MODES[self.mode],
+                                              self.s_max))
+
     def reserve(self, nbytes):
         """Allocates an arena of at least nbytes (drops a captured graph)."""
         if self.arena is None or self.arena.numel() < nbytes:
             self._create(nbytes)
+
+    def adopt_arena(self, arena):
+        """Uses the caller's contiguous uint8 CUDA tensor as the arena.  Several
+        samplers may share one arena if only one of them is used at a time
+        (each set_views() rewrites it)."""
+        if not (arena.is_cuda and arena.dtype == torch.uint8 and arena.is_contiguous()):
+            raise RuntimeError("MultiViewSampler: the arena must be a contiguous uint8 CUDA tensor")
+        self._create(arena=arena)
 
     def set_views(self, pairs):
         """pairs: list of (xyz [n,3] float32 CUDA tensor, camera)."""
@@ -164,6 +184,31 @@ class MultiViewSampler:
             out["nb_t"] = self._tensor(o.nb_t, R * S * K, torch.float32, (R, S, K))
             out["nb_dist"] = self._tensor(o.nb_dist, R * S * K, torch.float32, (R, S, K))
         return out
+
+    def batch_outputs(self):
+        """The compact outputs of all views as whole-batch tensors, and each
+        caller view's (ray_offset, rays) rows in them."""
+        rows = []
+        base = None
+        for i in range(len(self.views)):
+            o = _BatchViewOut()
+            _raise_on(_lib.epcq_batch_view_outputs(self.handle, int(i), ctypes.byref(o)))
+            rows.append((int(o.ray_offset), int(o.rays)))
+            if base is None:
+                base = o
+        T = self.rays
+        K = int(self.options.num_point_cloud_per_sp)
+        S = self.s_max
+        off = base.ray_offset
+        out = {
+            "n_samples": self._tensor(base.n_samples - off * 4, T, torch.int32, (T,)),
+            "sample_t": self._tensor(base.sample_t - off * S * 4, T * S, torch.float32, (T, S)),
+            "nb_idx": self._tensor(base.nb_idx - off * S * K * 4, T * S * K, torch.int32, (T, S, K)),
+        }
+        if base.nb_t:
+            out["nb_t"] = self._tensor(base.nb_t - off * S * K * 4, T * S * K, torch.float32, (T, S, K))
+            out["nb_dist"] = self._tensor(base.nb_dist - off * S * K * 4, T * S * K, torch.float32, (T, S, K))
+        return out, rows
 
     def close(self):
         if self.handle:

@@ -111,6 +111,8 @@ ABI_SYMBOLS = {
                                  _SZ, ctypes.POINTER(ctypes.c_void_p)]),
     "epcq_batch_set_views": (_I32, [_P, ctypes.POINTER(_ViewDesc), _I32, _P]),
     "epcq_batch_view_outputs": (_I32, [_P, _I32, ctypes.POINTER(_BatchViewOut)]),
+    "epcq_batch_header_bytes": (_SZ, [_I32]),
+    "epcq_batch_view_bytes": (_SZ, [ctypes.POINTER(_ViewDesc), ctypes.POINTER(_Options), _I32, _I32]),
     "epcq_batch_rays": (_I64, [_P]),
     "epcq_batch_run": (_I32, [_P, _P]),
     "epcq_batch_capture": (_I32, [_P, _P]),

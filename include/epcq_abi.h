@@ -229,6 +229,12 @@ typedef struct epcq_batch_view_out {
 /* Arena bytes for these views (capacity for up to max_views descriptors). */
 size_t epcq_batch_arena_bytes(const epcq_view_desc* views, int32_t n_views, int32_t max_views,
                               const epcq_options* opt, int32_t mode, int32_t s_max);
+/* Arena bytes of the header part (descriptors for max_views views), and an
+ * upper bound of the bytes one view adds (its build workspace, point list,
+ * hash table, bucket spheres and outputs): for any view set,
+ * arena_bytes <= header_bytes + sum of view_bytes.  Used for admission. */
+size_t epcq_batch_header_bytes(int32_t max_views);
+size_t epcq_batch_view_bytes(const epcq_view_desc* view, const epcq_options* opt, int32_t mode, int32_t s_max);
 /* radius: optional per-point radius of the (single) cloud, r_max its maximum;
  * tuning: launch shape (NULL = the process-wide setting). */
 int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t s_max, int32_t max_views,
