@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch shape, no autotuning")
     ap.add_argument("--no-serve", action="store_true", help="batched: skip the serving sweep")
+    ap.add_argument("--tuning", default=None, help="batched: launch shape tile_w,tile_h,tile_cap,flags (no autotuning)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     ap.add_argument("--plain-passes", type=int, default=3, help="batched: timed passes of the per-view forms")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     return ap.parse_args()
@@ -123,7 +126,10 @@ def bench_batched(args, wl):
     del probe
     arena = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     tune_log = {}
-    if args.no_tune:
+    if args.tuning:
+        tw, th, cap, fl = (int(x) for x in args.tuning.split(","))
+        best = {"tile_w": tw, "tile_h": th, "tile_cap": cap, "flags": fl}
+    elif args.no_tune:
         best = dict(autotune.DEFAULT)
     else:
         cands = {}
@@ -212,11 +218,16 @@ def bench_batched(args, wl):
         return (time.perf_counter() - t) / passes
 
     C.set_tuning(**best)
-    plain_s = timed(plain_pass, args.plain_passes)
-    compact_s = timed(compact_pass, args.plain_passes)
-    plain = rays_per_step / plain_s
-    compact = rays_per_step / compact_s
-    log(f"plain per-view drop-in: {plain_s * 1e3:.2f} ms/pass; per-view compact: {compact_s * 1e3:.2f} ms/pass")
+    if args.plain_passes > 0:
+        plain_s = timed(plain_pass, args.plain_passes)
+        compact_s = timed(compact_pass, args.plain_passes)
+        log(f"plain per-view drop-in: {plain_s * 1e3:.2f} ms/pass; per-view compact: {compact_s * 1e3:.2f} ms/pass")
+    else:
+        plain_s = compact_s = None
+    plain = rays_per_step / plain_s if plain_s else None
+    compact = rays_per_step / compact_s if compact_s else None
+    ratio = lambda x: value / x if x else None
+    ms = lambda x: x * 1e3 if x else None
 
     record = {
         "metric": METRIC,
@@ -236,12 +247,12 @@ def bench_batched(args, wl):
                    "radius": float(opt.radius_threshold), "K": K, "mode": args.mode, "s_max": s_max,
                    "form": "batched: one captured HIP graph per step", "tuning": best},
         "latency_per_step": percentiles(step_ms),
-        "plain_per_view": {"value": plain, "unit": "rays/s", "ms_per_pass": plain_s * 1e3,
+        "plain_per_view": {"value": plain, "unit": "rays/s", "ms_per_pass": ms(plain_s),
                            "calls": "pts2plane + render_image_hashtable_topk (dense outputs) per view",
-                           "batched_speedup": value / plain},
-        "per_view_compact": {"value": compact, "unit": "rays/s", "ms_per_pass": compact_s * 1e3,
+                           "batched_speedup": ratio(plain)},
+        "per_view_compact": {"value": compact, "unit": "rays/s", "ms_per_pass": ms(compact_s),
                              "calls": "pts2plane + knn_sampling (compact outputs) per view",
-                             "batched_speedup": value / compact},
+                             "batched_speedup": ratio(compact)},
         "stages_ms": stage,
         "samples_per_step": n_samples,
         "roofline": {"bound": "hbm", "kernel": "batch_sample_kernel + batch_fallback_kernel",
@@ -492,8 +503,24 @@ def cpu_baseline_views(wl, opt, target_s, mode, max_views=4):
                       f"{total} rays; oracle with {threads} OpenMP threads"}
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of `mib` MiB, five times: the reference
+    point that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
+    import torch
+    n = mib << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
               "(views would be sharded round-robin across GPUs, one process per GPU)", file=sys.stderr)
