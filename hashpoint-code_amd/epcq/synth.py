@@ -158,10 +158,29 @@ def box_scene(n, n_boxes=20, seed=0, jitter=0.002, plane_z=-1.0, plane_half=2.0)
     return pts.astype(np.float32)
 
 
+def exp_series(x):
+    """exp(x) for float64 arrays, |x| < 700: x = k ln2 + r with |r| <= ln2 / 2,
+    exp(r) from its Taylor series, times 2^k (exact)."""
+    x = np.asarray(x, dtype=np.float64)
+    k = np.floor(x / _LN2 + 0.5)
+    r = x - k * _LN2
+    term = np.ones_like(r)
+    acc = np.ones_like(r)
+    for j in range(1, 22):
+        term = term * r / j
+        acc = acc + term
+    return np.ldexp(acc, k.astype(np.int64))
+
+
+def _dot3(a, b):
+    return float(a[0]) * float(b[0]) + float(a[1]) * float(b[1]) + float(a[2]) * float(b[2])
+
+
 def log_uniform_radius(n, seed, lo=1e-3, hi=1e-2):
-    """Per-point radius exp(U[ln lo, ln hi]) as lo * (hi/lo)^u."""
+    """Per-point radius exp(U[ln lo, ln hi]) = lo * exp(u ln(hi / lo))."""
     u = Stream(seed, 3).uniform(n)
-    return (lo * np.power(hi / lo, u)).astype(np.float32)
+    span = float(log_series(np.array([hi / lo]))[0])
+    return (lo * exp_series(u * span)).astype(np.float32)
 
 
 @dataclass
@@ -180,16 +199,17 @@ class Camera:
 def look_at_camera(eye, width, height, focal):
     """Camera at `eye` looking at the origin (OpenCV axes)."""
     eye = np.asarray(eye, dtype=np.float64)
-    fwd = -eye / math.sqrt(float(eye @ eye))
+    fwd = -eye / math.sqrt(_dot3(eye, eye))
     up = np.array([0.0, 0.0, 1.0])
-    if abs(float(fwd @ up)) > 0.999:
+    if abs(_dot3(fwd, up)) > 0.999:
         up = np.array([0.0, 1.0, 0.0])
     right = np.cross(fwd, up)
-    right = right / math.sqrt(float(right @ right))
+    right = right / math.sqrt(_dot3(right, right))
     down = np.cross(fwd, right)
     R = np.stack([right, down, fwd], axis=1)  # columns: camera axes in world
     c2w = np.concatenate([R, eye[:, None]], axis=1)
-    w2c = np.concatenate([R.T, (-R.T @ eye)[:, None]], axis=1)
+    t = np.array([-_dot3(R[:, i], eye) for i in range(3)])
+    w2c = np.concatenate([R.T, t[:, None]], axis=1)
     K = np.array([[focal, 0.0, width / 2.0], [0.0, focal, height / 2.0], [0.0, 0.0, 1.0]])
     return Camera(c2w.astype(np.float32), w2c.astype(np.float32), K.astype(np.float32), float(np.float32(focal)),
                   float(np.float32(focal)), float(np.float32(width / 2.0)), float(np.float32(height / 2.0)),
