@@ -314,12 +314,12 @@ def bench_serving(args, fractions=(0.3, 0.6, 0.9), seconds=4.0, tuning=None, clo
 
     # warm: capture every slot's graph
     warm = poisson_trace(1e9, 4 * len(server.slots), clouds, seed=19)
-    run_trace(server, warm)
+    run_trace(server, warm, keep_results=False)
     # capacity: everything queued at once
     burst = poisson_trace(1e9, 400, clouds, seed=21)
     burst = [(0.0, c, cam) for _, c, cam in burst]
     t0 = time.perf_counter()
-    reqs = run_trace(server, burst)
+    reqs = run_trace(server, burst, keep_results=False)
     wall = time.perf_counter() - t0
     cap = summarize(reqs, wall)
     log(f"serving capacity: {cap['req_per_s']:.0f} req/s, {cap['rays_per_s'] / 1e9:.3f} G rays/s")
@@ -329,7 +329,7 @@ def bench_serving(args, fractions=(0.3, 0.6, 0.9), seconds=4.0, tuning=None, clo
         count = int(max(200, min(4000, rate * seconds)))
         trace = poisson_trace(rate, count, clouds, seed=20 + 10 * i)
         t0 = time.perf_counter()
-        reqs = run_trace(server, trace)
+        reqs = run_trace(server, trace, keep_results=False)
         wall = time.perf_counter() - t0
         r = summarize(reqs, wall)
         r["rate_req_per_s"] = rate

@@ -13,9 +13,10 @@
 // Pipeline (all views at once, one launch per stage):
 //   reset -> count -> col_partial -> bin_scan -> col_apply -> bin -> fine
 //   (the fused hash-table build of epcq_build.hip, per (view, chunk) / (view,
-//   bin) item) -> bounds (per-pixel bucket spheres) -> sample (one
-//   (view, tile) item at a time from an atomic work counter, heaviest views
-//   first) -> fallback (tiles whose halo exceeded the LDS capacity).
+//   bin) item) -> bounds (per-pixel bucket cones, pixel-ordered points) ->
+//   sample (one (view, tile) item at a time from an atomic work counter,
+//   heaviest views first) -> fallback (tiles whose halo exceeded the LDS
+//   capacity).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -140,8 +141,9 @@ __global__ __launch_bounds__(kBuildBlock) void batch_bounds_kernel(const BatchHe
         const BatchView& bv = views[v];
         const int64_t p = (int64_t)(item - prefix[v]) * kBuildBlock + threadIdx.x;
         if (p < bv.b.P)
-            bucket_bounds_body(bv.view.hash, bv.view.sorted_idx, bv.b.xyz, radius, r0, p,
-                               const_cast<float4*>(bv.view.spheres));
+            bucket_cone_body(bv.view.hash, bv.view.sorted_idx, bv.b.xyz, radius, r0, bv.view.c2w, p,
+                             const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b),
+                             const_cast<float4*>(bv.view.pts));
     }
 }
 
@@ -231,7 +233,7 @@ size_t header_bytes(int max_views) {
 struct ViewLayout {
     BuildPlan bp;
     int64_t P;
-    size_t ws, idx, hash, sph;  // byte offsets relative to the view's block
+    size_t ws, idx, hash, ca, cb, pts;  // byte offsets relative to the view's block
     size_t bytes;
 };
 
@@ -248,7 +250,9 @@ ViewLayout layout_view(const epcq_view_desc& d) {
     L.ws = take(L.bp.bytes);
     L.idx = take((size_t)d.n * 4);
     L.hash = take((size_t)L.P * 8);
-    L.sph = take((size_t)L.P * 16);
+    L.ca = take((size_t)L.P * 16);
+    L.cb = take((size_t)L.P * 8);
+    L.pts = take((size_t)d.n * 16);
     L.bytes = off;
     return L;
 }
@@ -486,7 +490,9 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         bv.b = make_build_view(L[i].bp, d.xyz, d.n, d_w2c, d_K, d.height, d.width, 0, vb + L[i].ws, out_idx, nullptr,
                                out_hash);
         bv.view = ViewArgs{d_c2w, d.fx, d.fy, d.cx, d.cy, d.width, d.height, reinterpret_cast<const int2*>(out_hash),
-                           out_idx, reinterpret_cast<const float4*>(vb + L[i].sph), d.xyz};
+                           out_idx, reinterpret_cast<const float4*>(vb + L[i].ca),
+                           reinterpret_cast<const float2*>(vb + L[i].cb), reinterpret_cast<const float4*>(vb + L[i].pts),
+                           d.xyz};
         bv.tiles_x = (int)ceil_div(d.width, b->tp.tw);
         bv.tiles = bv.tiles_x * (int)ceil_div(d.height, b->tp.th);
         bv.col_groups = (int)ceil_div(L[i].bp.nbins, kBuildBlock);

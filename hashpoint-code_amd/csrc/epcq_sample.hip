@@ -46,7 +46,7 @@ TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags) {
     p.rw_max = p.tw + k - 1;
     p.rh_max = p.th + k - 1;
     const size_t fixed =
-        align_up((size_t)p.rw_max * p.rh_max * (16 + 8) + (size_t)(2 * p.rh_max + 1) * 4, 16);
+        align_up((size_t)p.rw_max * p.rh_max * (16 + 8 + 8) + (size_t)(2 * p.rh_max + 1) * 4, 16);
     const size_t limit = 64 * 1024;
     size_t c = (size_t)std::max(cap, 0);
     if (fixed + c * 16 > limit) c = fixed < limit ? (limit - fixed) / 16 : 0;
@@ -98,17 +98,19 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
     Ray ray;
     view_ray(view, u, v, ray);
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.spheres, view.W};
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
     run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
 }
 
-__global__ __launch_bounds__(kBlock) void bucket_bounds_kernel(const int2* __restrict__ hash,
-                                                               const int* __restrict__ sorted_idx,
-                                                               const float* __restrict__ xyz,
-                                                               const float* __restrict__ radius, float r0, int64_t P,
-                                                               float4* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restrict__ hash,
+                                                             const int* __restrict__ sorted_idx,
+                                                             const float* __restrict__ xyz,
+                                                             const float* __restrict__ radius, float r0,
+                                                             const float* __restrict__ c2w, int64_t P,
+                                                             float4* __restrict__ cone_a, float2* __restrict__ cone_b,
+                                                             float4* __restrict__ pts) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p < P) bucket_bounds_body(hash, sorted_idx, xyz, radius, r0, p, out);
+    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, radius, r0, c2w, p, cone_a, cone_b, pts);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each
@@ -226,21 +228,33 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     return o;
 }
 
-ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash, const float4* spheres) {
+ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash) {
     return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
-                    reinterpret_cast<const int2*>(hash), sorted_idx, spheres, nullptr};
+                    reinterpret_cast<const int2*>(hash), sorted_idx, nullptr, nullptr, nullptr, nullptr};
 }
 
-// Bucket bounds into the workspace (when it is large enough); returns the
-// sphere array or nullptr (no culling).
-const float4* prepare_bounds(const SampleOpts& o, const epcq_camera* cam, const int* sorted_idx, const int* hash,
-                             void* workspace, size_t workspace_bytes, hipStream_t st) {
+// Workspace layout of the samplers: cone_a [P] float4 | cone_b [P] float2 |
+// pixel-ordered points [n] float4.
+size_t cone_bytes(int64_t P) { return align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256); }
+
+// Bucket cones and the pixel-ordered point copy into the workspace (when it
+// is large enough: then o.cull is set and view reads them).
+void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera* cam, const int* sorted_idx,
+                    const int* hash, void* workspace, size_t workspace_bytes, hipStream_t st) {
     const int64_t P = (int64_t)cam->width * cam->height;
-    if (!workspace || workspace_bytes < (size_t)P * sizeof(float4)) return nullptr;
-    float4* sph = static_cast<float4*>(workspace);
-    bucket_bounds_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
-        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, P, sph);
-    return sph;
+    o.cull = 0;
+    if (!workspace || workspace_bytes < cone_bytes(P)) return;
+    char* w = static_cast<char*>(workspace);
+    float4* ca = reinterpret_cast<float4*>(w);
+    float2* cb = reinterpret_cast<float2*>(w + align_up((size_t)P * 16, 256));
+    float4* pts = workspace_bytes >= cone_bytes(P) + (size_t)n * 16 ? reinterpret_cast<float4*>(w + cone_bytes(P))
+                                                                     : nullptr;
+    bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
+        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts);
+    o.cull = 1;
+    view.cone_a = ca;
+    view.cone_b = cb;
+    view.pts = pts;
 }
 
 template <int MODE, int KMAX, int SMAX>
@@ -339,9 +353,9 @@ extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max) {
     return visits;
 }
 
-extern "C" size_t epcq_sampling_workspace_size(int32_t height, int32_t width) {
-    if (height <= 0 || width <= 0) return 0;
-    return (size_t)height * width * sizeof(float4);
+extern "C" size_t epcq_sampling_workspace_size(int64_t n, int32_t height, int32_t width) {
+    if (height <= 0 || width <= 0 || n < 0) return 0;
+    return cone_bytes((int64_t)height * width) + (size_t)n * 16;
 }
 
 extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
@@ -355,7 +369,6 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
         return fail(EPCQ_ERR_INVALID, "quick_sampling: negative output size");
     if (mode == EPCQ_MODE_KNN && opt->num_point_cloud_per_sp > 32)
         return fail(EPCQ_ERR_INVALID, "quick_sampling: num_point_cloud_per_sp > 32 in KNN mode");
-    (void)n;
     if (opt->num_sample_points_per_ray == 0) return EPCQ_OK;
     int bound;
     if (int e = live_bound(opt, opt->radius_threshold, "quick_sampling", &bound)) return e;
@@ -366,10 +379,10 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
     out.sp_t = sp_t;
     out.mask = mask;
     hipStream_t st = as_stream(stream);
-    const float4* sph = prepare_bounds(o, cam, sorted_idx, hash, workspace, workspace_bytes, st);
-    o.cull = sph != nullptr;
-    return launch_sample(o, out, make_view(cam, sorted_idx, hash, sph), nullptr, (int64_t)cam->width * cam->height,
-                         mode, bound, st, "quick_sampling");
+    ViewArgs view = make_view(cam, sorted_idx, hash);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    return launch_sample(o, out, view, nullptr, (int64_t)cam->width * cam->height, mode, bound, st,
+                         "quick_sampling");
 }
 
 extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
@@ -387,7 +400,6 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     if (!n_samples || !sample_t || !nb_idx) return fail(EPCQ_ERR_INVALID, "knn_sampling: null output");
     if (feat && (F < 1 || !w || !agg)) return fail(EPCQ_ERR_INVALID, "knn_sampling: bad feature arguments");
     if (rays && R < 0) return fail(EPCQ_ERR_INVALID, "knn_sampling: negative ray count");
-    (void)n;
     // with per-point radii the caller's s_max stands for the bound
     int bound = s_max;
     if (!radius) {
@@ -411,9 +423,9 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     out.agg = agg;
     const int64_t nr = rays ? R : (int64_t)cam->width * cam->height;
     hipStream_t st = as_stream(stream);
-    const float4* sph = prepare_bounds(o, cam, sorted_idx, hash, workspace, workspace_bytes, st);
-    o.cull = sph != nullptr;
-    return launch_sample(o, out, make_view(cam, sorted_idx, hash, sph), rays, nr, mode, bound, st, "knn_sampling");
+    ViewArgs view = make_view(cam, sorted_idx, hash);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    return launch_sample(o, out, view, rays, nr, mode, bound, st, "knn_sampling");
 }
 
 extern "C" int epcq_quick_query(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,

@@ -73,15 +73,20 @@ struct SampleOut {
     float* agg;
 };
 
+// Per-pixel bucket cone (see bucket_cone_body): ca = (axis, P), cb = (Q, B).
 struct SrcGlobal {
     const int2* hash;
     const int* sorted_idx;
     const float* xyz;
-    const float4* spheres;  // per-pixel bucket bounds (when culling)
+    const float4* pts;   // pixel-ordered (x, y, z, index bits) or nullptr (gather)
+    const float4* cone_a;
+    const float2* cone_b;
     int W;
     __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[vb * W + ub]; }
-    __device__ __forceinline__ float4 sphere(int ub, int vb) const { return spheres[vb * W + ub]; }
+    __device__ __forceinline__ float4 cone4(int ub, int vb) const { return cone_a[vb * W + ub]; }
+    __device__ __forceinline__ float2 cone2(int ub, int vb) const { return cone_b[vb * W + ub]; }
     __device__ __forceinline__ float4 point(int j) const {
+        if (pts) return pts[j];
         const int id = sorted_idx[j];
         const float* p = xyz + 3 * (int64_t)id;
         return make_float4(p[0], p[1], p[2], __int_as_float(id));
@@ -91,10 +96,12 @@ struct SrcGlobal {
 struct SrcLds {
     const int2* hash;
     const float4* pts;
-    const float4* spheres;
+    const float4* cone_a;
+    const float2* cone_b;
     int ra, va, RW;
     __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[(vb - va) * RW + (ub - ra)]; }
-    __device__ __forceinline__ float4 sphere(int ub, int vb) const { return spheres[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float4 cone4(int ub, int vb) const { return cone_a[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float2 cone2(int ub, int vb) const { return cone_b[(vb - va) * RW + (ub - ra)]; }
     __device__ __forceinline__ float4 point(int j) const { return pts[j]; }
 };
 
@@ -125,62 +132,85 @@ __device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, f
     return cs > rm * rm * ray.dd;
 }
 
-// Bucket culling.  sph = (c, B): every point p of the bucket has |p - c| <= R
-// and radius <= B - R.  The distance from p to the ray (or to a point on it)
-// is at least the distance from c minus R, and the canonical rounded
-// distances differ from the exact ones by far less than the margin
-// 3e-6 (1 + |c - o|_1 + 3 B + |o|_1), so a bucket is skipped only if none of
-// its points can pass the exact test.
-__device__ __forceinline__ bool bucket_far_from_ray(const Ray& ray, const float4& sph) {
-    const float w0 = sph.x - ray.o[0], w1 = sph.y - ray.o[1], w2 = sph.z - ray.o[2];
-    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
-    const float lim = sph.w + __builtin_fmaf(3e-6f, aw + 3.0f * sph.w, ray.omargin);
-    const float c0 = __builtin_fmaf(w1, ray.d[2], -(w2 * ray.d[1]));
-    const float c1 = __builtin_fmaf(w2, ray.d[0], -(w0 * ray.d[2]));
-    const float c2 = __builtin_fmaf(w0, ray.d[1], -(w1 * ray.d[0]));
-    const float cs = __builtin_fmaf(c2, c2, __builtin_fmaf(c1, c1, c0 * c0));
-    return cs > lim * lim * ray.dd;
+// Bucket culling by view cones.  Every ray of a view starts at the same
+// origin o, so a bucket is summarised by the cone of its points seen from o:
+// unit axis a, rho <= |p - o| and phi >= the angle between the line through p
+// and the axis, for every point p of the bucket.  For a ray with direction d
+// at (line) angle alpha to the axis, each point is at distance
+// |p - o| sin(alpha - phi) >= rho (sin alpha cos phi - cos alpha sin phi)
+// from the ray.  The cone stores P = rho cos phi and Q = rho sin phi rounded
+// towards that bound, and B = the bucket's largest radius plus 4e-6 rho; the
+// ray side's cos / sin errors (a few ulp of rho) are inside the 4e-6 rho, and
+// the rounded distances the exact tests compute differ from the exact
+// distance by at most ~eps (18 |p - o| + 4 |o| + 1) (see surely_far), which
+// the slack of 4e-6 rho over 18 eps rho and the origin margin cover for
+// every |p - o| >= rho.  So cone_gap(...) > margin implies that no point of
+// the bucket passes the exact test.
+__device__ __forceinline__ float cone_gap(const Ray& ray, const float4& ca, const float2& cb) {
+    const float c = fabsf(dot3(ray.d[0], ray.d[1], ray.d[2], ca.x, ca.y, ca.z));
+    const float x0 = ray.d[1] * ca.z - ray.d[2] * ca.y;
+    const float x1 = ray.d[2] * ca.x - ray.d[0] * ca.z;
+    const float x2 = ray.d[0] * ca.y - ray.d[1] * ca.x;
+    const float s = sqrtf(sqnorm3(x0, x1, x2));
+    const float nd = sqrtf(ray.dd);
+    return (s * ca.w - c * cb.x) / nd - cb.y;
 }
 
-__device__ __forceinline__ bool bucket_far_from_point(const float* sp, const float4& sph) {
-    const float w0 = sph.x - sp[0], w1 = sph.y - sp[1], w2 = sph.z - sp[2];
-    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
-    const float lim = sph.w + 3e-6f * (1.0f + aw + 3.0f * sph.w);
-    const float d2 = __builtin_fmaf(w2, w2, __builtin_fmaf(w1, w1, w0 * w0));
-    return d2 > lim * lim;
-}
+// Lanes (k*k window positions, clamped) of a ray whose bucket is non-empty
+// and not culled with margin `margin`; bit l for lane l.  Windows of more
+// than 64 lanes are not culled (every bit set; lanes >= 64 always walk).
+using LaneMask = uint64_t;
+
+__device__ __forceinline__ bool lane_on(LaneMask m, int l) { return l >= 64 || ((m >> l) & 1ull); }
 
 // dist < radius for a squared distance, exact (see the file comment).
 __device__ __forceinline__ bool within(const SampleOpts& o, float e2, int id) {
     return o.radius ? sqrtf(e2) < o.radius[id] : e2 < o.r2lim;
 }
 
-// Calls fn(ub, vb) for the k*k neighbour lanes in order: lane l is
+// Calls fn(ub, vb, l) for the k*k neighbour lanes l in order: lane l is
 // (du, dv) = (l % k - (ko - 1), (l + k) / k - ko), i.e. rows dv = 1 - ko ..
 // k - ko, columns du = 1 - ko .. k - ko, clamped to the image
 // (calculate_neighbor_uv, cuda_util.cuh:382-388).
 template <class Fn>
 __device__ __forceinline__ void for_each_lane(const SampleOpts& o, int u, int v, int W, int H, Fn&& fn) {
+    int l = 0;
     for (int dv = 1 - o.ko; dv <= o.k - o.ko; ++dv) {
         const int vb = min(max(v + dv, 0), H - 1);
         for (int du = 1 - o.ko; du <= o.k - o.ko; ++du) {
             const int ub = min(max(u + du, 0), W - 1);
-            fn(ub, vb);
+            fn(ub, vb, l);
+            ++l;
         }
     }
 }
 
-// Calls fn(t, e2) for each hit of the ray: the points of its neighbour
-// buckets within their radius, at most 48 per bucket, in lane then bucket
-// order (quick_sampling_kernel :125-155).  The next point is read before the
-// current one is tested.
-template <class Src, class Fn>
-__device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                                 int W, int H, Fn&& fn) {
-    for_each_lane(o, u, v, W, H, [&](int ub, int vb) {
+template <class Src>
+__device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
+                                              int W, int H, float margin) {
+    if (o.L > 64) return ~0ull;
+    LaneMask m = 0;
+    for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
-        if (o.cull && bucket_far_from_ray(ray, src.sphere(ub, vb))) return;
+        if (o.cull && cone_gap(ray, src.cone4(ub, vb), src.cone2(ub, vb)) > margin) return;
+        m |= 1ull << l;
+    });
+    return m;
+}
+
+// Calls fn(t, e2) for each hit of the ray: the points of its neighbour
+// buckets within their radius, at most 48 per bucket, in lane then bucket
+// order (quick_sampling_kernel :125-155), skipping the lanes off in `mask`
+// (empty or culled buckets, which hold no hit).  The next point is read
+// before the current one is tested.
+template <class Src, class Fn>
+__device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
+                                                 int W, int H, LaneMask mask, Fn&& fn) {
+    for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
+        if (!lane_on(mask, l)) return;
+        const int2 e = src.bucket(ub, vb);
+        if (e.y <= 0) return;
         int kept = 0;
         float4 nxt = src.point(e.x);
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
@@ -311,12 +341,15 @@ struct NbBuf {
 };
 
 // One walk over the candidates of the window feeding the buffers of the
-// first `ns` (<= NS) sample points sp[].  Compat mode walks the k*k lanes
-// with their border duplicates, like quick_search_nearby_pc (:39-47); KNN
-// mode walks the distinct window pixels.
+// first `ns` (<= NS) sample points sp[], skipping the lanes off in `mask`
+// (buckets with no point within reach of any sample point).  Compat mode
+// walks the k*k lanes with their border duplicates, like
+// quick_search_nearby_pc (:39-47); KNN mode walks the distinct window pixels
+// (lane of pixel (ub, vb): its unclamped window position).
 template <int MODE, int KMAX, int NS, class Src>
 __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
-                                               const float (&sp)[NS][3], int ns, NbBuf<MODE, KMAX> (&buf)[NS]) {
+                                               const float (&sp)[NS][3], int ns, LaneMask mask,
+                                               NbBuf<MODE, KMAX> (&buf)[NS]) {
     auto visit = [&](const float4& p) {
         const int id = __float_as_int(p.w);
 #pragma unroll
@@ -327,17 +360,10 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
             }
         }
     };
-    auto bucket = [&](int ub, int vb) {
+    auto bucket = [&](int ub, int vb, int l) {
+        if (!lane_on(mask, l)) return;
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
-        if (o.cull) {
-            const float4 sph = src.sphere(ub, vb);
-            bool far = true;
-#pragma unroll
-            for (int q = 0; q < NS; ++q)
-                if (q < ns && !bucket_far_from_point(sp[q], sph)) far = false;
-            if (far) return;
-        }
         float4 nxt = src.point(e.x);
         for (int j = 0; j < e.y; ++j) {
             const float4 p = nxt;
@@ -351,7 +377,7 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
         const int ua = max(0, u - (o.ko - 1)), uz = min(W - 1, u + o.k - o.ko);
         const int va = max(0, v + 1 - o.ko), vz = min(H - 1, v + o.k - o.ko);
         for (int vb = va; vb <= vz; ++vb)
-            for (int ub = ua; ub <= uz; ++ub) bucket(ub, vb);
+            for (int ub = ua; ub <= uz; ++ub) bucket(ub, vb, (vb - v + o.ko - 1) * o.k + (ub - u + o.ko - 1));
     }
 }
 
@@ -446,10 +472,12 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
     float ts_arr[SMAX];
 #pragma unroll
     for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
-    if (valid_ray) {
+    // lanes whose bucket may hold a hit (a ray with none is empty)
+    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin) : 0ull;
+    if (valid_ray && hmask != 0ull) {
         // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
         float tmin = 1e3f, tmax = -1e3f;
-        for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float) {
+        for_each_ray_hit(o, src, ray, u, v, W, H, hmask, [&](float t, float) {
             if (tmin > t) tmin = t;
             if (tmax < t) tmax = t;
         });
@@ -471,7 +499,7 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
             }
             int nt = 0;
             int limit = kMaxSlots;  // the table holds complete sums of every occupied slot below this
-            for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float e2) {
+            for_each_ray_hit(o, src, ray, u, v, W, H, hmask, [&](float t, float e2) {
                 const float qf = (t - tmin) / o.t_intvl;
                 if (!(qf > -1.0f && qf < (float)kMaxSlots)) return;
                 const int s = (int)qf;
@@ -545,7 +573,7 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
                     }
                     limit = from;
                     const int first = s;
-                    for_each_ray_hit(o, src, ray, u, v, W, H, [&](float t, float e2) {
+                    for_each_ray_hit(o, src, ray, u, v, W, H, hmask, [&](float t, float e2) {
                         const float qf = (t - tmin) / o.t_intvl;
                         if (!(qf > -1.0f && qf < (float)kMaxSlots)) return;
                         const int q = (int)qf;
@@ -599,6 +627,8 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
         for (int g = 0; g < nstore; g += NS) {
             float pend[NS][3];
             NbBuf<MODE, KMAX> buf[NS];
+            const int cnt = min(NS, nstore - g);
+            float tabs = 0.0f;
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
                 float tq = 0.0f;
@@ -608,10 +638,15 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
                 pend[q][0] = ray.o[0] + tq * ray.d[0];
                 pend[q][1] = ray.o[1] + tq * ray.d[1];
                 pend[q][2] = ray.o[2] + tq * ray.d[2];
+                if (q < cnt) tabs = fmaxf(tabs, fabsf(tq));
                 buf[q].init();
             }
-            const int cnt = min(NS, nstore - g);
-            neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, buf);
+            // A sample point o + t d rounds to within ~eps (2 |t| |d|_1 + |o|_1)
+            // of the ray and |sp - p| to a few ulp, so a bucket whose cone
+            // clears the ray by that much more holds no neighbour.
+            const float dl1 = fabsf(ray.d[0]) + fabsf(ray.d[1]) + fabsf(ray.d[2]);
+            const LaneMask nmask = lane_mask(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1);
+            neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
 #pragma unroll
             for (int q = 0; q < NS; ++q)
                 if (q < cnt) write_neighbours<MODE, KMAX>(o, out, ray, buf[q], ri, g + q);
@@ -676,8 +711,10 @@ struct ViewArgs {
     int W, H;
     const int2* hash;
     const int* sorted_idx;
-    const float4* spheres;  // bucket bounds or nullptr
-    const float* xyz;       // the view's cloud (nullptr: SampleOpts::xyz)
+    const float4* cone_a;  // bucket cones (bucket_cone_body) or nullptr
+    const float2* cone_b;
+    const float4* pts;     // pixel-ordered points (bucket_cone_body) or nullptr
+    const float* xyz;      // the view's cloud (nullptr: SampleOpts::xyz)
 };
 
 __device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
@@ -687,11 +724,13 @@ __device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& 
     make_ray(m, v.fx, v.fy, v.cx, v.cy, u, vv, ray);
 }
 
-// LDS of one tile: points [cap] | spheres [rw*rh] | hash [rw*rh] | rowlo | rowbase.
+// LDS of one tile: points [cap] | cone_a [rw*rh] | hash [rw*rh] | cone_b
+// [rw*rh] | rowlo | rowbase.
 struct TileSmem {
     float4* pts;
-    float4* sph;
+    float4* ca;
     int2* hash;
+    float2* cb;
     int* rowlo;
     int* rowbase;
     int* total;
@@ -699,10 +738,12 @@ struct TileSmem {
 
 __device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan& tp, int* total) {
     TileSmem m;
+    const int rr = tp.rw_max * tp.rh_max;
     m.pts = dyn;
-    m.sph = dyn + tp.cap;
-    m.hash = reinterpret_cast<int2*>(m.sph + tp.rw_max * tp.rh_max);
-    m.rowlo = reinterpret_cast<int*>(m.hash + tp.rw_max * tp.rh_max);
+    m.ca = dyn + tp.cap;
+    m.hash = reinterpret_cast<int2*>(m.ca + rr);
+    m.cb = reinterpret_cast<float2*>(m.hash + rr);
+    m.rowlo = reinterpret_cast<int*>(m.cb + rr);
     m.rowbase = m.rowlo + tp.rh_max;
     m.total = total;
     return m;
@@ -748,70 +789,77 @@ __device__ __forceinline__ int region_points(const int2* __restrict__ hash, int 
     return *s_sum;
 }
 
-// Stages the halo region of the tile with top-left ray (u0, v0) into LDS.
-// Returns the number of staged points, or -1 if the region holds more than
-// `cap` points (the tile then reads global memory).  Block-wide; every
-// thread of the block must call it.
-__device__ __forceinline__ int stage_tile(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
-                                          const float* __restrict__ xyz, const float4* __restrict__ spheres,
-                                          int W, int H, int k, int ko, int u0, int v0, int tw, int th, int cap,
-                                          float4* s_pts, int2* s_hash, float4* s_sph, int* s_rowlo,
-                                          int* s_rowbase, int* s_total, int& ra, int& va, int& RW) {
+// Stages the halo region of the tile with top-left ray (u0, v0) into LDS:
+// its hash entries (rebased to the staged list), bucket cones (when
+// culling) and points.  Returns the number of staged points, or -1 if the
+// region holds more than `cap` points (the tile then reads global memory).
+// Block-wide; every thread of the block must call it.
+__device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __restrict__ xyz, bool cull, int k,
+                                          int ko, int u0, int v0, int tw, int th, int cap, const TileSmem& sm,
+                                          int& ra, int& va, int& RW) {
     int RH;
-    tile_region(W, H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
+    const int W = view.W;
+    tile_region(W, view.H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     for (int i = tid; i < RW * RH; i += nthr) {
         const int64_t g = (int64_t)(va + i / RW) * W + ra + i % RW;
-        s_hash[i] = hash[g];
-        if (spheres) s_sph[i] = spheres[g];
+        sm.hash[i] = view.hash[g];
+        if (cull) {
+            sm.ca[i] = view.cone_a[g];
+            sm.cb[i] = view.cone_b[g];
+        }
     }
     __syncthreads();
     for (int r = tid; r < RH; r += nthr) {
         int lo = 0, hi = 0;
         bool found = false;
         for (int c = 0; c < RW; ++c) {
-            const int2 e = s_hash[r * RW + c];
+            const int2 e = sm.hash[r * RW + c];
             if (e.y > 0) {
                 if (!found) lo = e.x;
                 found = true;
                 hi = e.x + e.y;
             }
         }
-        s_rowlo[r] = lo;
-        s_rowbase[r] = hi - lo;
+        sm.rowlo[r] = lo;
+        sm.rowbase[r] = hi - lo;
     }
     __syncthreads();
     if (tid == 0) {
         int run = 0;
         for (int r = 0; r < RH; ++r) {
-            const int len = s_rowbase[r];
-            s_rowbase[r] = run;
+            const int len = sm.rowbase[r];
+            sm.rowbase[r] = run;
             run += len;
         }
-        s_rowbase[RH] = run;
-        *s_total = run;
+        sm.rowbase[RH] = run;
+        *sm.total = run;
     }
     __syncthreads();
-    const int total = *s_total;
+    const int total = *sm.total;
     if (total > cap) return -1;
     for (int i = tid; i < total; i += nthr) {
         int lo = 0, hi = RH;  // last row r with rowbase[r] <= i
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (s_rowbase[mid] <= i) lo = mid;
+            if (sm.rowbase[mid] <= i) lo = mid;
             else hi = mid;
         }
-        const int gi = s_rowlo[lo] + (i - s_rowbase[lo]);
-        const int id = sorted_idx[gi];
-        const float* p = xyz + 3 * (int64_t)id;
-        s_pts[i] = make_float4(p[0], p[1], p[2], __int_as_float(id));
+        const int gi = sm.rowlo[lo] + (i - sm.rowbase[lo]);
+        if (view.pts) {
+            sm.pts[i] = view.pts[gi];
+        } else {
+            const int id = view.sorted_idx[gi];
+            const float* p = xyz + 3 * (int64_t)id;
+            sm.pts[i] = make_float4(p[0], p[1], p[2], __int_as_float(id));
+        }
     }
     for (int i = tid; i < RW * RH; i += nthr) {
-        const int2 e = s_hash[i];
+        const int2 e = sm.hash[i];
         if (e.y > 0) {
             const int r = i / RW;
-            s_hash[i].x = s_rowbase[r] + (e.x - s_rowlo[r]);
+            sm.hash[i].x = sm.rowbase[r] + (e.x - sm.rowlo[r]);
         }
     }
     __syncthreads();
@@ -830,15 +878,13 @@ __device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const Sam
     const int u0 = (tile % tiles_x) * tp.tw;
     const int v0 = (tile / tiles_x) * tp.th;
     int ra, va, RW;
-    const int staged = stage_tile(view.hash, view.sorted_idx, o.xyz, o.cull ? view.spheres : nullptr, view.W,
-                                  view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm.pts, sm.hash, sm.sph,
-                                  sm.rowlo, sm.rowbase, sm.total, ra, va, RW);
+    const int staged = stage_tile(view, o.xyz, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va, RW);
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
     if (staged >= 0 && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcLds src{sm.hash, sm.pts, sm.sph, ra, va, RW};
+        const SrcLds src{sm.hash, sm.pts, sm.ca, sm.cb, ra, va, RW};
         run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
     }
     __syncthreads();  // the LDS is reused by the block's next tile
@@ -859,42 +905,81 @@ __device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, 
     if (total > tp.cap && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.spheres, view.W};
+        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
         run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
     }
     __syncthreads();
 }
 
-// Bounding sphere of every pixel's points: (centre, R + largest radius), R
-// rounded up so that |p - centre| <= R for each point p; w = -1 for empty
-// pixels.  One thread per pixel.
-__device__ __forceinline__ void bucket_bounds_body(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
-                                                   const float* __restrict__ xyz, const float* __restrict__ radius,
-                                                   float r0, int64_t p, float4* __restrict__ out) {
+// Cone of pixel p's points seen from the view origin o (see cone_gap), and
+// the pixel's points copied in list order as (x, y, z, index bits) into pts
+// (when not null).  One thread per pixel; two passes over the bucket: the
+// axis (the normalised sum of the unit directions, each oriented to the
+// first one's side, since cones bound lines), then the largest angle to it.
+// A bucket holding a point at o or a non-finite coordinate gets a cone that
+// never culls.
+__device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
+                                                 const float* __restrict__ xyz, const float* __restrict__ radius,
+                                                 float r0, const float* __restrict__ c2w, int64_t p,
+                                                 float4* __restrict__ cone_a, float2* __restrict__ cone_b,
+                                                 float4* __restrict__ pts) {
     const int2 e = hash[p];
+    const float never = __builtin_inff();
     if (e.y <= 0) {
-        out[p] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        cone_a[p] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        cone_b[p] = make_float2(0.0f, never);
         return;
     }
-    float mn0 = __builtin_inff(), mn1 = __builtin_inff(), mn2 = __builtin_inff();
-    float mx0 = -__builtin_inff(), mx1 = -__builtin_inff(), mx2 = -__builtin_inff();
-    float rb = r0;
+    const float o0 = c2w[3], o1 = c2w[7], o2 = c2w[11];
+    float rho = never, rb = r0;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    bool ok = true;
     for (int j = 0; j < e.y; ++j) {
         const int id = sorted_idx[e.x + j];
         const float* q = xyz + 3 * (int64_t)id;
-        mn0 = fminf(mn0, q[0]);
-        mn1 = fminf(mn1, q[1]);
-        mn2 = fminf(mn2, q[2]);
-        mx0 = fmaxf(mx0, q[0]);
-        mx1 = fmaxf(mx1, q[1]);
-        mx2 = fmaxf(mx2, q[2]);
+        const float x = q[0], y = q[1], z = q[2];
+        if (pts) pts[e.x + j] = make_float4(x, y, z, __int_as_float(id));
         if (radius) rb = j == 0 ? radius[id] : fmaxf(rb, radius[id]);
+        const float w0 = x - o0, w1 = y - o1, w2 = z - o2;
+        const float n = sqrtf(sqnorm3(w0, w1, w2));
+        if (!(n > 0.0f && n < never)) {
+            ok = false;
+            continue;
+        }
+        const float u0 = w0 / n, u1 = w1 / n, u2 = w2 / n;
+        if (j == 0) {
+            f0 = u0;
+            f1 = u1;
+            f2 = u2;
+        }
+        const float sg = dot3(u0, u1, u2, f0, f1, f2) < 0.0f ? -1.0f : 1.0f;
+        s0 = s0 + sg * u0;
+        s1 = s1 + sg * u1;
+        s2 = s2 + sg * u2;
+        rho = fminf(rho, n);
     }
-    const float c0 = (mn0 + mx0) * 0.5f, c1 = (mn1 + mx1) * 0.5f, c2 = (mn2 + mx2) * 0.5f;
-    const float h0 = (mx0 - mn0) * 0.5f, h1 = (mx1 - mn1) * 0.5f, h2 = (mx2 - mn2) * 0.5f;
-    const float R = sqrtf(sqnorm3(h0, h1, h2)) * (1.0f + 1e-6f) +
-                    1e-6f * (1.0f + fabsf(c0) + fabsf(c1) + fabsf(c2));
-    out[p] = make_float4(c0, c1, c2, R + rb);
+    const float sn = sqrtf(sqnorm3(s0, s1, s2));
+    if (!ok || !(sn > 0.0f) || !(rb <= never)) {
+        cone_a[p] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        cone_b[p] = make_float2(0.0f, never);
+        return;
+    }
+    const float a0 = s0 / sn, a1 = s1 / sn, a2 = s2 / sn;
+    float cmin = 1.0f, smax = 0.0f;
+    for (int j = 0; j < e.y; ++j) {
+        const int id = sorted_idx[e.x + j];
+        const float* q = xyz + 3 * (int64_t)id;
+        const float w0 = q[0] - o0, w1 = q[1] - o1, w2 = q[2] - o2;
+        const float n = sqrtf(sqnorm3(w0, w1, w2));
+        const float u0 = w0 / n, u1 = w1 / n, u2 = w2 / n;
+        cmin = fminf(cmin, fabsf(dot3(u0, u1, u2, a0, a1, a2)));
+        const float x0 = u1 * a2 - u2 * a1, x1 = u2 * a0 - u0 * a2, x2 = u0 * a1 - u1 * a0;
+        smax = fmaxf(smax, sqrtf(sqnorm3(x0, x1, x2)));
+    }
+    // rho, cos phi rounded down, sin phi up (a few ulp of error each)
+    const float rl = rho * (1.0f - 1e-6f);
+    cone_a[p] = make_float4(a0, a1, a2, rl * (cmin - 1e-6f));
+    cone_b[p] = make_float2(rl * (smax + 1e-6f), rb * (1.0f + 1e-6f) + 4e-6f * rho);
 }
 
 }  // namespace epcq

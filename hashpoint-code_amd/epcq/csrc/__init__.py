@@ -97,7 +97,7 @@ ABI_SYMBOLS = {
     "epcq_build_hashtable": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "epcq_quick_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
                                    _I32, _P, _P, _P, _P, _P, _SZ, _P]),
-    "epcq_sampling_workspace_size": (_SZ, [_I32, _I32]),
+    "epcq_sampling_workspace_size": (_SZ, [_I64, _I32, _I32]),
     "epcq_quick_query": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
                                 _I64, _P, _P]),
     "epcq_knn_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
@@ -387,10 +387,10 @@ def build_hashtable_workspace_size(n, height, width):
     return int(_lib.epcq_build_hashtable_workspace_size(int(n), int(height), int(width)))
 
 
-def _workspace(H, W, device, cull):
+def _workspace(n, H, W, device, cull):
     if not cull:
         return None
-    return torch.empty(int(_lib.epcq_sampling_workspace_size(H, W)), dtype=torch.uint8, device=device)
+    return torch.empty(int(_lib.epcq_sampling_workspace_size(n, H, W)), dtype=torch.uint8, device=device)
 
 
 def _nbytes(t):
@@ -414,7 +414,7 @@ def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable
     sp_t = torch.empty((H, W, S), dtype=torch.float32, device=device)
     pc_idx = torch.empty((H, W, S, Kc), dtype=torch.int32, device=device)
     mask = torch.empty((H, W, S), dtype=torch.bool, device=device)
-    ws = _workspace(H, W, device, cull)
+    ws = _workspace(xyz.size(0), H, W, device, cull)
     camera, opts = _camera_struct(cam), _options_struct(options)
     with torch.cuda.device(device):
         _raise_on(_lib.epcq_quick_sampling(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
@@ -527,7 +527,7 @@ def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, 
         bf16 = int(feat.dtype == torch.bfloat16)
         out["w"] = torch.empty((R, s_max, Kc), dtype=torch.float32, device=device)
         out["agg"] = torch.empty((R, s_max, F), dtype=torch.float32, device=device)
-    ws = _workspace(H, W, device, cull)
+    ws = _workspace(xyz.size(0), H, W, device, cull)
     camera, opts = _camera_struct(cam), _options_struct(options)
     with torch.cuda.device(device):
         _raise_on(_lib.epcq_knn_sampling(

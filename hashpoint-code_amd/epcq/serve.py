@@ -148,14 +148,14 @@ class PointServer:
 
     def poll(self):
         """Completes finished batches and starts new ones on idle slots.
-        Returns the number of requests completed by this call."""
-        done = 0
+        Returns the requests completed by this call."""
+        done = []
         for slot in self.slots:
             if slot.event is not None and slot.event.query():
                 now = self.clock()
                 for r in slot.requests:
                     r.finished = now
-                done += len(slot.requests)
+                done.extend(slot.requests)
                 slot.requests, slot.event = [], None
         for slot in self.slots:
             if slot.event is None and self.pending:
@@ -175,9 +175,11 @@ class PointServer:
             s.sampler.close()
 
 
-def run_trace(server, trace, idle_sleep=2e-5):
+def run_trace(server, trace, idle_sleep=2e-5, keep_results=True):
     """Replays a request trace [(t_arrival_s, cloud, cam), ...] (sorted by
-    time) on the server's clock and returns the requests."""
+    time) on the server's clock and returns the requests.  With
+    keep_results=False each result is dropped as soon as its request
+    completes (a client that consumed it), so memory stays bounded."""
     t0 = server.clock()
     reqs = []
     i, n = 0, len(trace)
@@ -187,7 +189,11 @@ def run_trace(server, trace, idle_sleep=2e-5):
             t, cloud, cam = trace[i]
             reqs.append(server.submit(cloud, cam, arrival=t0 + t))
             i += 1
-        if not server.poll() and not server.pending:
+        done = server.poll()
+        if not keep_results:
+            for r in done:
+                r.result = None
+        if not done and not server.pending:
             time.sleep(idle_sleep)
     return reqs
 

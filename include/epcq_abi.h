@@ -142,10 +142,12 @@ int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
                         int32_t mode, float* sp_xyz, int32_t* pc_idx, float* sp_t, uint8_t* mask,
                         void* workspace, size_t workspace_bytes, void* stream);
 
-/* Workspace of the samplers: per-pixel bucket bounds that let the walks skip
- * neighbour pixels provably outside the radius.  Optional: with a NULL or
- * smaller workspace the samplers skip nothing (same results). */
-size_t epcq_sampling_workspace_size(int32_t height, int32_t width);
+/* Workspace of the samplers for n points and an H x W image: per-pixel
+ * bucket cones that let the walks skip neighbour pixels provably outside the
+ * radius, and a pixel-ordered copy of the points.  Optional: with a NULL or
+ * smaller workspace the samplers skip nothing and gather the points through
+ * sorted_idx (same results). */
+size_t epcq_sampling_workspace_size(int64_t n, int32_t height, int32_t width);
 
 /*
  * quick_query_for_nearby_point -- replaces `quick_query_for_nearby_point`
