@@ -13,7 +13,8 @@ namespace epcq {
 constexpr int kBuildBlock = 256;
 constexpr int kBuildWaves = kBuildBlock / kWave;
 constexpr int kMaxBinBits = 13;    // <= 8192 coarse bins (32 KB of LDS cursors)
-constexpr int kMinTileShift = 10;  // >= 1024 pixels per bin
+constexpr int kMaxTileShift = 10;  // <= 1024 pixels per bin unless the image needs more
+constexpr int kBinPoints = 2048;   // target points per bin (dense views get smaller bins)
 constexpr int kSegBlocks = 64;     // blocks per column-scan segment
 
 struct ProjParams {
@@ -94,8 +95,16 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& tot
     return wbase + x - v;
 }
 
-// Exclusive scan of a[0..len) in LDS (len a multiple of 256), in place.
+// Exclusive scan of a[0..len) in LDS (len a power of two), in place.
 __device__ void lds_exclusive_scan(int* a, int len, int* s_wave) {
+    if (len < kBuildBlock) {
+        const int v = (int)threadIdx.x < len ? a[threadIdx.x] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, s_wave, total);
+        if ((int)threadIdx.x < len) a[threadIdx.x] = ex;
+        __syncthreads();
+        return;
+    }
     const int per = len / kBuildBlock;
     int* seg = a + threadIdx.x * per;
     int sum = 0;
@@ -170,7 +179,13 @@ struct BuildPlan {
 inline BuildPlan plan_build(int64_t n, int H, int W) {
     BuildPlan bp{};
     bp.P = (int64_t)H * W;
-    bp.ts = std::max(kMinTileShift, ceil_log2(bp.P) - kMaxBinBits);
+    // bins of at most 2^kMaxTileShift pixels, fewer for dense views so that a
+    // bin holds about kBinPoints points or less (the fine pass gives each bin
+    // one block), but never more than 2^kMaxBinBits bins
+    const int64_t q = (int64_t)kBinPoints * bp.P / std::max<int64_t>(n, 1);
+    int ts_dense = 0;
+    while (ts_dense < kMaxTileShift && (int64_t(2) << ts_dense) <= q) ++ts_dense;
+    bp.ts = std::max(ceil_log2(bp.P) - kMaxBinBits, ts_dense);
     bp.nbins = (int)ceil_div(bp.P, int64_t(1) << bp.ts);
     bp.bin_bits = std::max(1, ceil_log2(bp.nbins));
     // about 2048 blocks at most, each a multiple of 4096 points
