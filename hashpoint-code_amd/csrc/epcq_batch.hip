@@ -315,6 +315,7 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     SampleOpts o{};
     o.k = b->opt.kernel_size;
     o.ko = (b->opt.kernel_size + 1) / 2;
+    o.kdiv = (65536 + b->opt.kernel_size - 1) / b->opt.kernel_size;
     o.L = o.k * o.k;
     o.r0 = b->opt.radius_threshold;
     o.r2lim = squared_threshold(b->opt.radius_threshold);

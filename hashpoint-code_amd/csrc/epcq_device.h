@@ -25,6 +25,7 @@ struct Ray {
     float d[3];
     float dd;       // dot(d, d), the divisor of PointRayInterset
     float omargin;  // 3e-6 (1 + |o|_1): origin part of the rejection margin
+    float nd;       // sqrt(dd)
 };
 
 __device__ __forceinline__ float dot3(float a0, float a1, float a2, float b0, float b1, float b2) {
@@ -74,6 +75,7 @@ __device__ __forceinline__ void make_ray(const float* __restrict__ m, float fx, 
     r.o[2] = m[11];
     r.dd = dot3(r.d[0], r.d[1], r.d[2], r.d[0], r.d[1], r.d[2]);
     r.omargin = 3e-6f * (1.0f + fabsf(r.o[0]) + fabsf(r.o[1]) + fabsf(r.o[2]));
+    r.nd = sqrtf(r.dd);
 }
 
 // SingleRaySpec::PointRayInterset (cuda_util.cuh:215-236): t along the ray and

@@ -214,6 +214,7 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     SampleOpts o{};
     o.k = opt->kernel_size;
     o.ko = (opt->kernel_size + 1) / 2;
+    o.kdiv = (65536 + opt->kernel_size - 1) / opt->kernel_size;
     o.L = opt->kernel_size * opt->kernel_size;
     o.r0 = opt->radius_threshold;
     o.r2lim = squared_threshold(opt->radius_threshold);

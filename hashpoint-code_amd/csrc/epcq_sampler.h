@@ -45,6 +45,7 @@ constexpr int kSlotTable = 8;
 
 struct SampleOpts {
     int k, ko, L;         // window size, ceil(k/2), k*k
+    int kdiv;             // ceil(65536 / k): l / k == (l * kdiv) >> 16 for lanes l < 64
     float r0, r2lim;      // uniform radius and its exact squared threshold
     const float* radius;  // per-point radius or nullptr
     float t_intvl, alpha_p, gamma;
@@ -144,29 +145,29 @@ __device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, f
 // the rounded distances the exact tests compute differ from the exact
 // distance by at most ~eps (18 |p - o| + 4 |o| + 1) (see surely_far), which
 // the slack of 4e-6 rho over 18 eps rho and the origin margin cover for
-// every |p - o| >= rho.  So cone_gap(...) > margin implies that no point of
-// the bucket passes the exact test.
-__device__ __forceinline__ float cone_gap(const Ray& ray, const float4& ca, const float2& cb) {
+// every |p - o| >= rho.  So a bucket clears the ray by `margin` -- cone gap
+// (s P - c Q) / |d| - B > margin, with c = |d . a| and s = |d x a| -- only if
+// no point of the bucket passes the exact test.  cone_clears evaluates the
+// equivalent s^2 P^2 > (c Q + (B + margin) |d|)^2 (for P > 0) without square
+// roots or divisions; its few extra roundings stay far inside the slack.
+__device__ __forceinline__ bool cone_clears(const Ray& ray, const float4& ca, const float2& cb, float margin) {
+    if (!(ca.w > 0.0f)) return false;
     const float c = fabsf(dot3(ray.d[0], ray.d[1], ray.d[2], ca.x, ca.y, ca.z));
     const float x0 = ray.d[1] * ca.z - ray.d[2] * ca.y;
     const float x1 = ray.d[2] * ca.x - ray.d[0] * ca.z;
     const float x2 = ray.d[0] * ca.y - ray.d[1] * ca.x;
-    const float s = sqrtf(sqnorm3(x0, x1, x2));
-    const float nd = sqrtf(ray.dd);
-    return (s * ca.w - c * cb.x) / nd - cb.y;
+    const float sq = sqnorm3(x0, x1, x2);
+    const float x = c * cb.x + (cb.y + margin) * ray.nd;
+    return sq * (ca.w * ca.w) > x * x;
 }
-
-// Lanes (k*k window positions, clamped) of a ray whose bucket is non-empty
-// and not culled with margin `margin`; bit l for lane l.  Windows of more
-// than 64 lanes are not culled (every bit set; lanes >= 64 always walk).
-using LaneMask = uint64_t;
-
-__device__ __forceinline__ bool lane_on(LaneMask m, int l) { return l >= 64 || ((m >> l) & 1ull); }
 
 // dist < radius for a squared distance, exact (see the file comment).
 __device__ __forceinline__ bool within(const SampleOpts& o, float e2, int id) {
     return o.radius ? sqrtf(e2) < o.radius[id] : e2 < o.r2lim;
 }
+
+// Lane sets of one ray: bit l for window lane l (k*k lanes, at most 64).
+using LaneMask = uint64_t;
 
 // Calls fn(ub, vb, l) for the k*k neighbour lanes l in order: lane l is
 // (du, dv) = (l % k - (ko - 1), (l + k) / k - ko), i.e. rows dv = 1 - ko ..
@@ -185,17 +186,69 @@ __device__ __forceinline__ void for_each_lane(const SampleOpts& o, int u, int v,
     }
 }
 
+// Calls fn(ub, vb, l) for the lanes set in `mask`, in lane order.  Windows of
+// more than 64 lanes carry no masks: every lane is visited.
+template <class Fn>
+__device__ __forceinline__ void for_each_masked_lane(const SampleOpts& o, int u, int v, int W, int H, LaneMask mask,
+                                                     Fn&& fn) {
+    if (o.L > 64) {
+        for_each_lane(o, u, v, W, H, fn);
+        return;
+    }
+    while (mask) {
+        const int l = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1ull;
+        const int r = (l * o.kdiv) >> 16;
+        const int ub = min(max(u + (l - r * o.k) - (o.ko - 1), 0), W - 1);
+        const int vb = min(max(v + r - (o.ko - 1), 0), H - 1);
+        fn(ub, vb, l);
+    }
+}
+
+// Lanes whose bucket is non-empty (emask) and, of those, the ones whose cone
+// does not clear the ray by `margin` (returned; every lane when culling is
+// off).  Only the returned lanes can hold hits.
 template <class Src>
 __device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                              int W, int H, float margin) {
-    if (o.L > 64) return ~0ull;
-    LaneMask m = 0;
+                                              int W, int H, float margin, LaneMask& emask) {
+    if (o.L > 64) {
+        emask = ~0ull;
+        return ~0ull;
+    }
+    LaneMask m = 0, e = 0;
     for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
-        const int2 e = src.bucket(ub, vb);
-        if (e.y <= 0) return;
-        if (o.cull && cone_gap(ray, src.cone4(ub, vb), src.cone2(ub, vb)) > margin) return;
+        if (src.bucket(ub, vb).y <= 0) return;
+        e |= 1ull << l;
+        if (o.cull && cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) return;
         m |= 1ull << l;
     });
+    emask = e;
+    return m;
+}
+
+// The lanes of `cand` whose cone does not clear the ray by `margin`.
+template <class Src>
+__device__ __forceinline__ LaneMask recheck_lanes(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
+                                                  int W, int H, float margin, LaneMask cand) {
+    LaneMask m = 0;
+    for_each_masked_lane(o, u, v, W, H, cand, [&](int ub, int vb, int l) {
+        if (!cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) m |= 1ull << l;
+    });
+    return m;
+}
+
+// Lanes whose unclamped window position lies in the image: one lane per
+// distinct window pixel.
+__device__ __forceinline__ LaneMask inside_lanes(const SampleOpts& o, int u, int v, int W, int H) {
+    LaneMask cols = 0, m = 0;
+    for (int c = 0; c < o.k; ++c) {
+        const int uu = u + c - (o.ko - 1);
+        if (uu >= 0 && uu < W) cols |= 1ull << c;
+    }
+    for (int r = 0; r < o.k; ++r) {
+        const int vv = v + r - (o.ko - 1);
+        if (vv >= 0 && vv < H) m |= cols << (r * o.k);
+    }
     return m;
 }
 
@@ -207,8 +260,7 @@ __device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& sr
 template <class Src, class Fn>
 __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
                                                  int W, int H, LaneMask mask, Fn&& fn) {
-    for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
-        if (!lane_on(mask, l)) return;
+    for_each_masked_lane(o, u, v, W, H, mask, [&](int ub, int vb, int) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
         int kept = 0;
@@ -341,11 +393,11 @@ struct NbBuf {
 };
 
 // One walk over the candidates of the window feeding the buffers of the
-// first `ns` (<= NS) sample points sp[], skipping the lanes off in `mask`
-// (buckets with no point within reach of any sample point).  Compat mode
-// walks the k*k lanes with their border duplicates, like
+// first `ns` (<= NS) sample points sp[], over the lanes set in `mask`
+// (buckets that may hold a point within reach of a sample point).  Compat
+// mode walks the k*k lanes with their border duplicates, like
 // quick_search_nearby_pc (:39-47); KNN mode walks the distinct window pixels
-// (lane of pixel (ub, vb): its unclamped window position).
+// (the caller restricts `mask` to inside_lanes).
 template <int MODE, int KMAX, int NS, class Src>
 __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
                                                const float (&sp)[NS][3], int ns, LaneMask mask,
@@ -360,8 +412,7 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
             }
         }
     };
-    auto bucket = [&](int ub, int vb, int l) {
-        if (!lane_on(mask, l)) return;
+    auto bucket = [&](int ub, int vb, int) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
         float4 nxt = src.point(e.x);
@@ -371,13 +422,13 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
             visit(p);
         }
     };
-    if (MODE == EPCQ_MODE_COMPAT) {
-        for_each_lane(o, u, v, W, H, bucket);
+    if (MODE == EPCQ_MODE_COMPAT || o.L <= 64) {
+        for_each_masked_lane(o, u, v, W, H, mask, bucket);
     } else {
         const int ua = max(0, u - (o.ko - 1)), uz = min(W - 1, u + o.k - o.ko);
         const int va = max(0, v + 1 - o.ko), vz = min(H - 1, v + o.k - o.ko);
         for (int vb = va; vb <= vz; ++vb)
-            for (int ub = ua; ub <= uz; ++ub) bucket(ub, vb, (vb - v + o.ko - 1) * o.k + (ub - u + o.ko - 1));
+            for (int ub = ua; ub <= uz; ++ub) bucket(ub, vb, 0);
     }
 }
 
@@ -473,7 +524,8 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
 #pragma unroll
     for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
     // lanes whose bucket may hold a hit (a ray with none is empty)
-    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin) : 0ull;
+    LaneMask emask = 0;
+    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin, emask) : 0ull;
     if (valid_ray && hmask != 0ull) {
         // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
         float tmin = 1e3f, tmax = -1e3f;
@@ -645,7 +697,10 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
             // of the ray and |sp - p| to a few ulp, so a bucket whose cone
             // clears the ray by that much more holds no neighbour.
             const float dl1 = fabsf(ray.d[0]) + fabsf(ray.d[1]) + fabsf(ray.d[2]);
-            const LaneMask nmask = lane_mask(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1);
+            LaneMask nmask = hmask;
+            if (o.cull && o.L <= 64)
+                nmask |= recheck_lanes(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1, emask & ~hmask);
+            if (MODE != EPCQ_MODE_COMPAT && o.L <= 64) nmask &= inside_lanes(o, u, v, W, H);
             neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
 #pragma unroll
             for (int q = 0; q < NS; ++q)
