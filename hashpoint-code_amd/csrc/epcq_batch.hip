@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
         if (item >= n_items) break;
         const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
-        sample_tile_body<MODE, KMAX, SMAX>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x, bv.ray_off, sm);
+        sample_tile_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x, bv.ray_off,
+                                                        sm);
     }
 }
 
@@ -181,8 +182,8 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchHeader* 
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
-        sample_tile_fallback_body<MODE, KMAX, SMAX>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x, bv.ray_off,
-                                                    &s_sum);
+        sample_tile_fallback_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
+                                                                 bv.ray_off, &s_sum);
     }
 }
 

@@ -68,27 +68,27 @@ epcq_tuning current_tuning() {
     return g_tuning;
 }
 
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
                                                              int tiles_x) {
     extern __shared__ float4 s_dyn[];
     __shared__ int s_total;
     const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total);
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    sample_tile_body<MODE, KMAX, SMAX>(o, out, view, tp, tile, tiles_x, 0, sm);
+    sample_tile_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm);
 }
 
 // The tiles sample_tile_kernel skipped (halo region above the LDS capacity),
 // sampled from global memory.
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 __global__ __launch_bounds__(kBlock) void sample_tile_fallback_kernel(SampleOpts o, SampleOut out, ViewArgs view,
                                                                       TilePlan tp, int tiles_x) {
     __shared__ int s_sum;
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    sample_tile_fallback_body<MODE, KMAX, SMAX>(o, out, view, tp, tile, tiles_x, 0, &s_sum);
+    sample_tile_fallback_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, &s_sum);
 }
 
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, SampleOut out, ViewArgs view,
                                                              const int* __restrict__ rays, int64_t R) {
     const int64_t ri = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     Ray ray;
     view_ray(view, u, v, ray);
     const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
-    run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
+    run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
 }
 
 __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restrict__ hash,
@@ -258,27 +258,35 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     view.pts = pts;
 }
 
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                 const TilePlan& tp, hipStream_t st) {
     if (rays) {
-        sample_list_kernel<MODE, KMAX, SMAX><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
+        sample_list_kernel<MODE, KMAX, SMAX, LAY><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
             o, out, view, rays, R);
     } else {
         const int tiles_x = (int)ceil_div(view.W, tp.tw);
         const int tiles = tiles_x * (int)ceil_div(view.H, tp.th);
         const dim3 block(tp.tw * tp.th);
-        (void)kBlock;
-        sample_tile_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp, tiles_x);
-        sample_tile_fallback_kernel<MODE, KMAX, SMAX><<<dim3(tiles), block, 0, st>>>(o, out, view, tp, tiles_x);
+        sample_tile_kernel<MODE, KMAX, SMAX, LAY><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp,
+                                                                                          tiles_x);
+        sample_tile_fallback_kernel<MODE, KMAX, SMAX, LAY><<<dim3(tiles), block, 0, st>>>(o, out, view, tp, tiles_x);
     }
+}
+
+template <int MODE, int KMAX, int SMAX>
+void launch_lay(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                const TilePlan& tp, hipStream_t st) {
+    if (out.sp_xyz) launch_one<MODE, KMAX, SMAX, kLayDense>(o, out, view, rays, R, tp, st);
+    else if (o.feat) launch_one<MODE, KMAX, SMAX, kLayFeat>(o, out, view, rays, R, tp, st);
+    else launch_one<MODE, KMAX, SMAX, kLayCompact>(o, out, view, rays, R, tp, st);
 }
 
 template <int MODE, int KMAX>
 void launch_smax(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                  const TilePlan& tp, int bound, hipStream_t st) {
-    if (bound <= 2) launch_one<MODE, KMAX, 2>(o, out, view, rays, R, tp, st);
-    else launch_one<MODE, KMAX, kMaxLiveSamples>(o, out, view, rays, R, tp, st);
+    if (bound <= 2) launch_lay<MODE, KMAX, 2>(o, out, view, rays, R, tp, st);
+    else launch_lay<MODE, KMAX, kMaxLiveSamples>(o, out, view, rays, R, tp, st);
 }
 
 // `bound`: samples one ray can emit (<= kMaxLiveSamples, checked by callers).

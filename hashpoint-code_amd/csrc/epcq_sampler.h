@@ -75,6 +75,13 @@ struct SampleOut {
 };
 
 // Per-pixel bucket cone (see bucket_cone_body): ca = (axis, P), cb = (Q, B).
+// Output layouts, fixed per kernel instantiation: the dense drop-in arrays,
+// the compact arrays, or the compact arrays with inverse-distance weighted
+// features.
+constexpr int kLayDense = 0;
+constexpr int kLayCompact = 1;
+constexpr int kLayFeat = 2;
+
 struct SrcGlobal {
     const int2* hash;
     const int* sorted_idx;
@@ -433,11 +440,11 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
 }
 
 // Writes the neighbour outputs of sample slot `so` of ray `ri`.
-template <int MODE, int KMAX>
+template <int MODE, int KMAX, int LAY>
 __device__ __forceinline__ void write_neighbours(const SampleOpts& o, const SampleOut& out, const Ray& ray,
                                                  const NbBuf<MODE, KMAX>& nb, int64_t ri, int so) {
     const int nkeep = nb.bsize < o.Kc ? nb.bsize : o.Kc;
-    if (out.sp_xyz) {
+    if constexpr (LAY == kLayDense) {
         int* dst = out.pc_idx + (ri * o.S + so) * (int64_t)o.Kc;
 #pragma unroll
         for (int b = 0; b < KMAX; ++b)
@@ -445,6 +452,7 @@ __device__ __forceinline__ void write_neighbours(const SampleOpts& o, const Samp
         for (int b = KMAX; b < o.Kc; ++b) dst[b] = -1;
         return;
     }
+    // compact rows; neighbour depth and distance in KNN mode only
     const int64_t row = (ri * out.s_max + so) * (int64_t)o.Kc;
     float den = 0.0f;
     float inv[KMAX];
@@ -454,71 +462,78 @@ __device__ __forceinline__ void write_neighbours(const SampleOpts& o, const Samp
         if (b < o.Kc) {
             const bool valid = b < nkeep;
             out.nb_idx[row + b] = valid ? nb.bi[b] : -1;
-            if (out.nb_t || out.nb_dist) {
-                float t = 0.0f, dist = 0.0f;
-                if (valid) {
-                    const float* p = o.xyz + 3 * (int64_t)nb.bi[b];
-                    float e2;
-                    ray_t_e2(ray, make_float4(p[0], p[1], p[2], 0.0f), t, e2);
-                    dist = sqrtf(e2);
+            if constexpr (MODE == EPCQ_MODE_KNN) {
+                if (out.nb_t || out.nb_dist) {
+                    float t = 0.0f, dist = 0.0f;
+                    if (valid) {
+                        const float* p = o.xyz + 3 * (int64_t)nb.bi[b];
+                        float e2;
+                        ray_t_e2(ray, make_float4(p[0], p[1], p[2], 0.0f), t, e2);
+                        dist = sqrtf(e2);
+                    }
+                    if (out.nb_t) out.nb_t[row + b] = t;
+                    if (out.nb_dist) out.nb_dist[row + b] = dist;
                 }
-                if (out.nb_t) out.nb_t[row + b] = t;
-                if (out.nb_dist) out.nb_dist[row + b] = dist;
             }
-            if (valid) {
-                inv[b] = 1.0f / (nb.bd[b] + 1e-6f);  // (1 / (d + 1e-6)) * mask, epcq.py:436
-                den = den + inv[b];
+            if constexpr (LAY == kLayFeat) {
+                if (valid) {
+                    inv[b] = 1.0f / (nb.bd[b] + 1e-6f);  // (1 / (d + 1e-6)) * mask, epcq.py:436
+                    den = den + inv[b];
+                }
             }
         }
     }
     for (int b = KMAX; b < o.Kc; ++b) {
         out.nb_idx[row + b] = -1;
-        if (out.nb_t) out.nb_t[row + b] = 0.0f;
-        if (out.nb_dist) out.nb_dist[row + b] = 0.0f;
+        if constexpr (MODE == EPCQ_MODE_KNN) {
+            if (out.nb_t) out.nb_t[row + b] = 0.0f;
+            if (out.nb_dist) out.nb_dist[row + b] = 0.0f;
+        }
     }
-    if (!o.feat) return;
-    den = den + 1e-6f;
-    float* wo = out.w + row;
+    if constexpr (LAY == kLayFeat) {
+        den = den + 1e-6f;
+        float* wo = out.w + row;
 #pragma unroll
-    for (int b = 0; b < KMAX; ++b)
-        if (b < o.Kc) wo[b] = inv[b] / den;
-    for (int b = KMAX; b < o.Kc; ++b) wo[b] = 0.0f;
-    float* ao = out.agg + (ri * out.s_max + so) * (int64_t)o.F;
-    for (int f0 = 0; f0 < o.F; f0 += 4) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < KMAX; ++b)
+            if (b < o.Kc) wo[b] = inv[b] / den;
+        for (int b = KMAX; b < o.Kc; ++b) wo[b] = 0.0f;
+        float* ao = out.agg + (ri * out.s_max + so) * (int64_t)o.F;
+        for (int f0 = 0; f0 < o.F; f0 += 4) {
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int b = 0; b < KMAX; ++b) {
-            if (b < nkeep) {
-                const float wb = inv[b] / den;
-                const int64_t base = (int64_t)nb.bi[b] * o.F + f0;
+            for (int b = 0; b < KMAX; ++b) {
+                if (b < nkeep) {
+                    const float wb = inv[b] / den;
+                    const int64_t base = (int64_t)nb.bi[b] * o.F + f0;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    if (f0 + c < o.F) {
-                        float fv;
-                        if (o.feat_bf16) {
-                            const uint16_t hb = static_cast<const uint16_t*>(o.feat)[base + c];
-                            fv = __uint_as_float((uint32_t)hb << 16);
-                        } else {
-                            fv = static_cast<const float*>(o.feat)[base + c];
+                    for (int c = 0; c < 4; ++c) {
+                        if (f0 + c < o.F) {
+                            float fv;
+                            if (o.feat_bf16) {
+                                const uint16_t hb = static_cast<const uint16_t*>(o.feat)[base + c];
+                                fv = __uint_as_float((uint32_t)hb << 16);
+                            } else {
+                                fv = static_cast<const float*>(o.feat)[base + c];
+                            }
+                            acc[c] = acc[c] + wb * fv;
                         }
-                        acc[c] = acc[c] + wb * fv;
                     }
                 }
             }
-        }
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (f0 + c < o.F) ao[f0 + c] = acc[c];
+            for (int c = 0; c < 4; ++c)
+                if (f0 + c < o.F) ao[f0 + c] = acc[c];
+        }
     }
 }
 
 // Samples and neighbours of one ray; writes every output element of row ri.
 // SMAX bounds the samples one ray can emit (epcq_max_live_samples).
-template <int MODE, int KMAX, int SMAX, class Src>
+template <int MODE, int KMAX, int SMAX, int LAY, class Src>
 __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
                         int W, int H, int64_t ri, bool valid_ray) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;  // sample points served by one neighbour walk
-    const bool dense = out.sp_xyz != nullptr;
+    constexpr bool dense = LAY == kLayDense;
     int ns = 0;
     float ts_arr[SMAX];
 #pragma unroll
@@ -656,7 +671,7 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
                     const float s0 = ray.o[0] + ts * ray.d[0];
                     const float s1 = ray.o[1] + ts * ray.d[1];
                     const float s2 = ray.o[2] + ts * ray.d[2];
-                    if (dense) {
+                    if constexpr (dense) {
                         const int64_t at = ri * o.S + ns;
                         out.sp_xyz[3 * at + 0] = s0;
                         out.sp_xyz[3 * at + 1] = s1;
@@ -704,12 +719,12 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
             neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
 #pragma unroll
             for (int q = 0; q < NS; ++q)
-                if (q < cnt) write_neighbours<MODE, KMAX>(o, out, ray, buf[q], ri, g + q);
+                if (q < cnt) write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
         }
     }
 
     // Everything not written above: zeros / -1 / false.
-    if (dense) {
+    if constexpr (dense) {
         for (int s = ns; s < o.S; ++s) {
             const int64_t at = ri * o.S + s;
             out.sp_xyz[3 * at + 0] = 0.0f;
@@ -727,11 +742,13 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
             out.sample_t[at] = 0.0f;
             for (int b = 0; b < o.Kc; ++b) {
                 out.nb_idx[at * o.Kc + b] = -1;
-                if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
-                if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
-                if (o.feat) out.w[at * o.Kc + b] = 0.0f;
+                if constexpr (MODE == EPCQ_MODE_KNN) {
+                    if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
+                    if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
+                }
+                if constexpr (LAY == kLayFeat) out.w[at * o.Kc + b] = 0.0f;
             }
-            if (o.feat)
+            if constexpr (LAY == kLayFeat)
                 for (int f = 0; f < o.F; ++f) out.agg[at * o.F + f] = 0.0f;
         }
     }
@@ -924,7 +941,7 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
 // One tile of one view: stage its halo region and sample its rays.  Output
 // row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
 // persistent kernels can call it in a loop.
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 __device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                  const TileSmem& sm) {
@@ -940,13 +957,14 @@ __device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const Sam
         Ray ray;
         view_ray(view, u, v, ray);
         const SrcLds src{sm.hash, sm.pts, sm.ca, sm.cb, ra, va, RW};
-        run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
+        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
+                                       true);
     }
     __syncthreads();  // the LDS is reused by the block's next tile
 }
 
 // A tile whose halo region exceeded the LDS capacity, from global memory.
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int LAY>
 __device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, const SampleOut& out,
                                                           const ViewArgs& view, const TilePlan& tp, int tile,
                                                           int tiles_x, int64_t ray_off, int* s_sum) {
@@ -961,7 +979,8 @@ __device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, 
         Ray ray;
         view_ray(view, u, v, ray);
         const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
-        run_ray<MODE, KMAX, SMAX>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u, true);
+        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
+                                       true);
     }
     __syncthreads();
 }
