@@ -59,6 +59,8 @@ struct BatchHeader {
     int V;
     int items[kStages];
     int counter;    // next sample item
+    int ntail;      // tiles left to the fallback pass
+    int* tail;      // their items
     SampleOut out;  // compact outputs of the current views (moves with the view set)
 };
 
@@ -73,7 +75,10 @@ __device__ __forceinline__ int find_view(const int* __restrict__ prefix, int V, 
     return lo;
 }
 
-__global__ void batch_reset_kernel(BatchHeader* h) { h->counter = 0; }
+__global__ void batch_reset_kernel(BatchHeader* h) {
+    h->counter = 0;
+    h->ntail = 0;
+}
 
 __global__ __launch_bounds__(kBuildBlock) void batch_count_kernel(const BatchHeader* __restrict__ h,
                                                                   const BatchView* __restrict__ views,
@@ -166,8 +171,9 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
         if (item >= n_items) break;
         const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
-        sample_tile_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x, bv.ray_off,
-                                                        sm);
+        const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v],
+                                                                            bv.tiles_x, bv.ray_off, sm);
+        if (!staged && threadIdx.x == 0) h->tail[atomicAdd(&h->ntail, 1)] = item;
     }
 }
 
@@ -177,9 +183,10 @@ __global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchHeader* 
                                                              const int* __restrict__ prefix, SampleOpts o,
                                                              TilePlan tp) {
     __shared__ int s_sum;
-    const int n_items = h->items[kTile], V = h->V;
+    const int n_items = h->ntail, V = h->V;  // the tiles batch_sample_kernel listed
     const SampleOut out = h->out;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const int item = h->tail[i];
         const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
         sample_tile_fallback_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
@@ -258,10 +265,12 @@ ViewLayout layout_view(const epcq_view_desc& d) {
     return L;
 }
 
+// Compact outputs of `rays` rays plus the fallback tile list (at most one
+// tile per ray).
 size_t output_bytes(int64_t rays, int s_max, int K, int mode) {
-    size_t per = 4 + (size_t)s_max * (4 + 4 * K);
+    size_t per = 4 + 4 + (size_t)s_max * (4 + 4 * K);
     if (mode == EPCQ_MODE_KNN) per += (size_t)s_max * 8 * K;
-    return align_up((size_t)rays * per + 4 * 256, 256);
+    return align_up((size_t)rays * per + 8 * 256, 256);  // + the alignment of each array
 }
 
 int check_views(const epcq_view_desc* views, int n_views) {
@@ -444,6 +453,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         off = align_up(off + bytes, 256);
         return at;
     };
+    const size_t o_tail = take((size_t)rays * 4);
     const size_t o_ns = take((size_t)rays * 4);
     const size_t o_t = take((size_t)rays * b->s_max * 4);
     const size_t o_idx = take((size_t)rays * b->s_max * K * 4);
@@ -494,7 +504,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         bv.view = ViewArgs{d_c2w, d.fx, d.fy, d.cx, d.cy, d.width, d.height, reinterpret_cast<const int2*>(out_hash),
                            out_idx, reinterpret_cast<const float4*>(vb + L[i].ca),
                            reinterpret_cast<const float2*>(vb + L[i].cb), reinterpret_cast<const float4*>(vb + L[i].pts),
-                           d.xyz};
+                           d.xyz, nullptr};
         bv.tiles_x = (int)ceil_div(d.width, b->tp.tw);
         bv.tiles = bv.tiles_x * (int)ceil_div(d.height, b->tp.th);
         bv.col_groups = (int)ceil_div(L[i].bp.nbins, kBuildBlock);
@@ -517,6 +527,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
     b->header.V = V;
     for (int k = 0; k < kStages; ++k) b->header.items[k] = P[k * M + V];
     b->header.out = b->out;
+    b->header.tail = reinterpret_cast<int*>(A + o_tail);
     b->V = V;
     b->total_rays = rays;
     hipStream_t st = as_stream(stream);

@@ -787,6 +787,7 @@ struct ViewArgs {
     const float2* cone_b;
     const float4* pts;     // pixel-ordered points (bucket_cone_body) or nullptr
     const float* xyz;      // the view's cloud (nullptr: SampleOpts::xyz)
+    int* tail;             // [0] count, [1..] tiles left to the fallback pass; or nullptr
 };
 
 __device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
@@ -941,8 +942,10 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
 // One tile of one view: stage its halo region and sample its rays.  Output
 // row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
 // persistent kernels can call it in a loop.
+// Returns false (block-uniform) when the tile's region did not fit the LDS
+// and the tile is left to the fallback pass.
 template <int MODE, int KMAX, int SMAX, int LAY>
-__device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
+__device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                  const TileSmem& sm) {
     SampleOpts o = o0;
@@ -961,6 +964,7 @@ __device__ __forceinline__ void sample_tile_body(const SampleOpts& o0, const Sam
                                        true);
     }
     __syncthreads();  // the LDS is reused by the block's next tile
+    return staged >= 0;
 }
 
 // A tile whose halo region exceeded the LDS capacity, from global memory.
