@@ -15,8 +15,8 @@
 //   (the fused hash-table build of epcq_build.hip, per (view, chunk) / (view,
 //   bin) item) -> bounds (per-pixel bucket cones, pixel-ordered points) ->
 //   sample (one (view, tile) item at a time from an atomic work counter,
-//   heaviest views first) -> fallback (tiles whose halo exceeded the LDS
-//   capacity).
+//   heaviest views first; a tile whose halo exceeds the LDS reads global
+//   memory).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,8 +59,6 @@ struct BatchHeader {
     int V;
     int items[kStages];
     int counter;    // next sample item
-    int ntail;      // tiles left to the fallback pass
-    int* tail;      // their items
     SampleOut out;  // compact outputs of the current views (moves with the view set)
 };
 
@@ -75,10 +73,7 @@ __device__ __forceinline__ int find_view(const int* __restrict__ prefix, int V, 
     return lo;
 }
 
-__global__ void batch_reset_kernel(BatchHeader* h) {
-    h->counter = 0;
-    h->ntail = 0;
-}
+__global__ void batch_reset_kernel(BatchHeader* h) { h->counter = 0; }
 
 __global__ __launch_bounds__(kBuildBlock) void batch_count_kernel(const BatchHeader* __restrict__ h,
                                                                   const BatchView* __restrict__ views,
@@ -171,26 +166,14 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
         if (item >= n_items) break;
         const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
+        // a tile whose halo region exceeds the LDS reads global memory right
+        // away: the densest views come first, so these long tiles overlap
+        // with the rest of the batch
         const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v],
                                                                             bv.tiles_x, bv.ray_off, sm);
-        if (!staged && threadIdx.x == 0) h->tail[atomicAdd(&h->ntail, 1)] = item;
-    }
-}
-
-template <int MODE, int KMAX, int SMAX>
-__global__ __launch_bounds__(256) void batch_fallback_kernel(const BatchHeader* __restrict__ h,
-                                                             const BatchView* __restrict__ views,
-                                                             const int* __restrict__ prefix, SampleOpts o,
-                                                             TilePlan tp) {
-    __shared__ int s_sum;
-    const int n_items = h->ntail, V = h->V;  // the tiles batch_sample_kernel listed
-    const SampleOut out = h->out;
-    for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
-        const int item = h->tail[i];
-        const int v = find_view(prefix, V, item);
-        const BatchView& bv = views[v];
-        sample_tile_fallback_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
-                                                                 bv.ray_off, &s_sum);
+        if (!staged)
+            sample_tile_global<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
+                                                              bv.ray_off);
     }
 }
 
@@ -265,10 +248,9 @@ ViewLayout layout_view(const epcq_view_desc& d) {
     return L;
 }
 
-// Compact outputs of `rays` rays plus the fallback tile list (at most one
-// tile per ray).
+// Compact outputs of `rays` rays.
 size_t output_bytes(int64_t rays, int s_max, int K, int mode) {
-    size_t per = 4 + 4 + (size_t)s_max * (4 + 4 * K);
+    size_t per = 4 + (size_t)s_max * (4 + 4 * K);
     if (mode == EPCQ_MODE_KNN) per += (size_t)s_max * 8 * K;
     return align_up((size_t)rays * per + 8 * 256, 256);  // + the alignment of each array
 }
@@ -292,8 +274,6 @@ void enqueue_sample(epcq_batch* b, const SampleOpts& o, hipStream_t st) {
     const BatchView* dv = b->d_views;
     const int* pt = b->d_prefix + kTile * (b->max_views + 1);  // tiles, densest views first
     batch_sample_kernel<MODE, KMAX, SMAX><<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(
-        b->d_header, dv, pt, o, b->tp);
-    batch_fallback_kernel<MODE, KMAX, SMAX><<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), 0, st>>>(
         b->d_header, dv, pt, o, b->tp);
 }
 
@@ -453,7 +433,6 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         off = align_up(off + bytes, 256);
         return at;
     };
-    const size_t o_tail = take((size_t)rays * 4);
     const size_t o_ns = take((size_t)rays * 4);
     const size_t o_t = take((size_t)rays * b->s_max * 4);
     const size_t o_idx = take((size_t)rays * b->s_max * K * 4);
@@ -527,7 +506,6 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
     b->header.V = V;
     for (int k = 0; k < kStages; ++k) b->header.items[k] = P[k * M + V];
     b->header.out = b->out;
-    b->header.tail = reinterpret_cast<int*>(A + o_tail);
     b->V = V;
     b->total_rays = rays;
     hipStream_t st = as_stream(stream);

@@ -967,6 +967,23 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     return staged >= 0;
 }
 
+// Every ray of a tile, reading global memory (no LDS, no barriers).
+template <int MODE, int KMAX, int SMAX, int LAY>
+__device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
+                                                   const TilePlan& tp, int tile, int tiles_x, int64_t ray_off) {
+    SampleOpts o = o0;
+    if (view.xyz) o.xyz = view.xyz;
+    const int u = (tile % tiles_x) * tp.tw + (int)threadIdx.x % tp.tw;
+    const int v = (tile / tiles_x) * tp.th + (int)threadIdx.x / tp.tw;
+    if (u < view.W && v < view.H) {
+        Ray ray;
+        view_ray(view, u, v, ray);
+        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
+        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
+                                       true);
+    }
+}
+
 // A tile whose halo region exceeded the LDS capacity, from global memory.
 template <int MODE, int KMAX, int SMAX, int LAY>
 __device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, const SampleOut& out,
