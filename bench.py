@@ -178,7 +178,7 @@ def bench_batched(args, wl):
     # stage split of the same pass (eager, events between stages on its stream)
     prof = [sampler.profile() for _ in range(max(3, min(args.steps, 10)))]
     torch.cuda.synchronize()
-    stage = {k: float(np.median([p[k] for p in prof])) for k in ("build", "bounds", "sample")}
+    stage = {k: float(np.median([p[k] for p in prof])) for k in ("bin", "place", "build", "sample")}
     n_in = 0
     n_samples = 0
     for i in range(V):

@@ -119,31 +119,37 @@ __global__ __launch_bounds__(kBuildBlock) void batch_bin_kernel(const BatchHeade
     }
 }
 
+// fine, then for the same bin: the pixel-ordered copy of its points and the
+// cones of its pixels (the samplers' bucket bounds), while they are in cache.
 __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHeader* __restrict__ h,
                                                                  const BatchView* __restrict__ views,
-                                                                 const int* __restrict__ prefix) {
+                                                                 const int* __restrict__ prefix,
+                                                                 const float* __restrict__ radius, float r0) {
     __shared__ int s_mem[2 * kMaxFineTile];
     __shared__ int s_wave[kBuildWaves];
     const int n_items = h->items[kFine], V = h->V;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int v = find_view(prefix, V, item);
-        fine_body(views[v].b, item - prefix[v], s_mem, s_wave);
-    }
-}
-
-__global__ __launch_bounds__(kBuildBlock) void batch_bounds_kernel(const BatchHeader* __restrict__ h,
-                                                                   const BatchView* __restrict__ views,
-                                                                   const int* __restrict__ prefix,
-                                                                   const float* __restrict__ radius, float r0) {
-    const int n_items = h->items[kPix], V = h->V;  // items of 256 pixels
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const int v = find_view(prefix, V, item);
         const BatchView& bv = views[v];
-        const int64_t p = (int64_t)(item - prefix[v]) * kBuildBlock + threadIdx.x;
-        if (p < bv.b.P)
-            bucket_cone_body(bv.view.hash, bv.view.sorted_idx, bv.b.xyz, radius, r0, bv.view.c2w, p,
-                             const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b),
-                             const_cast<float4*>(bv.view.pts));
+        const int b = item - prefix[v];
+        fine_body(bv.b, b, s_mem, s_wave);
+        const int bs = bv.b.bstart[b], tot = bv.b.btotal[b];
+        float4* pts = const_cast<float4*>(bv.view.pts);
+        const float* xyz = bv.b.xyz;
+        for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock) {
+            const int id = bv.b.out_idx[i];
+            const float* q = xyz + 3 * (int64_t)id;
+            pts[i] = make_float4(q[0], q[1], q[2], __int_as_float(id));
+        }
+        __syncthreads();
+        const int64_t p0 = (int64_t)b << bv.b.ts;
+        for (int f = threadIdx.x; f < (1 << bv.b.ts); f += kBuildBlock) {
+            const int64_t p = p0 + f;
+            if (p < bv.b.P)
+                bucket_cone_body(bv.view.hash, nullptr, nullptr, pts, radius, r0, bv.view.c2w, p,
+                                 const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b), nullptr);
+        }
+        __syncthreads();
     }
 }
 
@@ -284,7 +290,8 @@ void enqueue_sample_smax(epcq_batch* b, const SampleOpts& o, hipStream_t st) {
 }
 
 // Enqueues the pass; ev (optional, 4 events) brackets the stages
-// build | bounds | sample.
+// projection and binning | per-pixel placement with the bucket cones |
+// sampling.
 int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     const int M = b->max_views + 1;
     BatchHeader* h = b->d_header;
@@ -298,9 +305,8 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     batch_binscan_kernel<<<gb, blk, 0, st>>>(h, dv);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 1);
     batch_bin_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
-    batch_fine_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kFine * M);
     if (ev) hipEventRecord(ev[1], st);
-    batch_bounds_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kPix * M, b->radius, b->opt.radius_threshold);
+    batch_fine_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kFine * M, b->radius, b->opt.radius_threshold);
     if (ev) hipEventRecord(ev[2], st);
     SampleOpts o{};
     o.k = b->opt.kernel_size;

@@ -342,7 +342,7 @@ __device__ __forceinline__ void bin_body(const BuildView& bv, int blk, int* s_cu
         const int pos = ordered_append(s_cur, bin, bv.bin_bits, active);
         if (active) {
             bv.tidx[pos] = (int)i;
-            bv.tdep[pos] = depth;
+            if (bv.out_depth) bv.tdep[pos] = depth;  // depths only when they are an output
             bv.tfine[pos] = (uint16_t)((uint32_t)key & fmask);
         }
     }

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restr
                                                              float4* __restrict__ pts, int* __restrict__ tail) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p == 0 && tail) tail[0] = 0;  // the samplers' fallback list starts empty
-    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, radius, r0, c2w, p, cone_a, cone_b, pts);
+    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, nullptr, radius, r0, c2w, p, cone_a, cone_b, pts);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each

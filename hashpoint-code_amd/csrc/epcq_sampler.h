@@ -1006,16 +1006,18 @@ __device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, 
     __syncthreads();
 }
 
-// Cone of pixel p's points seen from the view origin o (see cone_gap), and
-// the pixel's points copied in list order as (x, y, z, index bits) into pts
-// (when not null).  One thread per pixel; two passes over the bucket: the
-// axis (the normalised sum of the unit directions, each oriented to the
-// first one's side, since cones bound lines), then the largest angle to it.
-// A bucket holding a point at o or a non-finite coordinate gets a cone that
-// never culls.
+// Cone of pixel p's points seen from the view origin o (see cone_gap).  The
+// points come from pts_in (pixel-ordered (x, y, z, index bits)) when given,
+// else through sorted_idx / xyz, in which case they are also copied in list
+// order into pts (when not null).  One thread per pixel; two passes over the
+// bucket: the axis (the normalised sum of the unit directions, each oriented
+// to the first one's side, since cones bound lines), then the largest angle
+// to it.  A bucket holding a point at o or a non-finite coordinate gets a cone
+// that never culls.
 __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
-                                                 const float* __restrict__ xyz, const float* __restrict__ radius,
-                                                 float r0, const float* __restrict__ c2w, int64_t p,
+                                                 const float* __restrict__ xyz, const float4* __restrict__ pts_in,
+                                                 const float* __restrict__ radius, float r0,
+                                                 const float* __restrict__ c2w, int64_t p,
                                                  float4* __restrict__ cone_a, float2* __restrict__ cone_b,
                                                  float4* __restrict__ pts) {
     const int2 e = hash[p];
@@ -1025,17 +1027,22 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
         cone_b[p] = make_float2(0.0f, never);
         return;
     }
+    auto point = [&](int j) {
+        if (pts_in) return pts_in[e.x + j];
+        const int id = sorted_idx[e.x + j];
+        const float* q = xyz + 3 * (int64_t)id;
+        return make_float4(q[0], q[1], q[2], __int_as_float(id));
+    };
     const float o0 = c2w[3], o1 = c2w[7], o2 = c2w[11];
     float rho = never, rb = r0;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
     bool ok = true;
     for (int j = 0; j < e.y; ++j) {
-        const int id = sorted_idx[e.x + j];
-        const float* q = xyz + 3 * (int64_t)id;
-        const float x = q[0], y = q[1], z = q[2];
-        if (pts) pts[e.x + j] = make_float4(x, y, z, __int_as_float(id));
+        const float4 q = point(j);
+        const int id = __float_as_int(q.w);
+        if (!pts_in && pts) pts[e.x + j] = q;
         if (radius) rb = j == 0 ? radius[id] : fmaxf(rb, radius[id]);
-        const float w0 = x - o0, w1 = y - o1, w2 = z - o2;
+        const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
         const float n = sqrtf(sqnorm3(w0, w1, w2));
         if (!(n > 0.0f && n < never)) {
             ok = false;
@@ -1062,9 +1069,8 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
     const float a0 = s0 / sn, a1 = s1 / sn, a2 = s2 / sn;
     float cmin = 1.0f, smax = 0.0f;
     for (int j = 0; j < e.y; ++j) {
-        const int id = sorted_idx[e.x + j];
-        const float* q = xyz + 3 * (int64_t)id;
-        const float w0 = q[0] - o0, w1 = q[1] - o1, w2 = q[2] - o2;
+        const float4 q = pts_in ? pts_in[e.x + j] : (pts ? pts[e.x + j] : point(j));
+        const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
         const float n = sqrtf(sqnorm3(w0, w1, w2));
         const float u0 = w0 / n, u1 = w1 / n, u2 = w2 / n;
         cmin = fminf(cmin, fabsf(dot3(u0, u1, u2, a0, a1, a2)));

@@ -152,11 +152,12 @@ class MultiViewSampler:
 
     def profile(self):
         """Runs the pass eagerly, timing its stages on the sampler's stream:
-        {"build": ms, "bounds": ms, "sample": ms}."""
+        {"bin": ms (projection and binning), "place": ms (per-pixel placement,
+        hash table and bucket cones), "build": bin + place, "sample": ms}."""
         self.stream.wait_stream(torch.cuda.current_stream(self.device))
         ms = (ctypes.c_float * 3)()
         _raise_on(_lib.epcq_batch_profile(self.handle, ctypes.c_void_p(self.stream.cuda_stream), ms))
-        return {"build": ms[0], "bounds": ms[1], "sample": ms[2]}
+        return {"bin": ms[0], "place": ms[1], "build": ms[0] + ms[1], "sample": ms[2]}
 
     def _tensor(self, ptr, count, dtype, shape):
         if not ptr:

@@ -251,7 +251,8 @@ int epcq_batch_run(epcq_batch* b, void* stream);      /* enqueue the pass */
 int epcq_batch_capture(epcq_batch* b, void* stream);  /* capture the pass into a HIP graph */
 int epcq_batch_launch(epcq_batch* b, void* stream);   /* replay the graph */
 /* Runs the pass eagerly with HIP events between its stages and waits for it:
- * stage_ms[0..2] = build, bucket bounds, sampling (milliseconds). */
+ * stage_ms[0..2] = projection and binning, per-pixel placement (hash table,
+ * point lists, bucket cones), sampling (milliseconds). */
 int epcq_batch_profile(epcq_batch* b, void* stream, float* stage_ms);
 void epcq_batch_destroy(epcq_batch* b);
 
