@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched", choices=["batched", "serving", "box", "sphere", "large"])
-    ap.add_argument("--mode", default="compat", choices=["compat", "knn"])
+    ap.add_argument("--mode", default=None, choices=["compat", "knn"],
+                    help="neighbour mode (default: compat, knn for the large workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch shape, no autotuning")
     ap.add_argument("--no-serve", action="store_true", help="batched: skip the serving sweep")
@@ -518,6 +519,8 @@ def calibrate_copy(mib):
 
 def main():
     args = parse()
+    if args.mode is None:
+        args.mode = "knn" if args.workload == "large" else "compat"
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return

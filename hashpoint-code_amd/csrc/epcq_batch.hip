@@ -489,7 +489,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         bv.view = ViewArgs{d_c2w, d.fx, d.fy, d.cx, d.cy, d.width, d.height, reinterpret_cast<const int2*>(out_hash),
                            out_idx, reinterpret_cast<const float4*>(vb + L[i].ca),
                            reinterpret_cast<const float2*>(vb + L[i].cb), reinterpret_cast<const float4*>(vb + L[i].pts),
-                           d.xyz, nullptr};
+                           d.xyz};
         bv.tiles_x = (int)ceil_div(d.width, b->tp.tw);
         bv.tiles = bv.tiles_x * (int)ceil_div(d.height, b->tp.th);
         bv.col_groups = (int)ceil_div(L[i].bp.nbins, kBuildBlock);

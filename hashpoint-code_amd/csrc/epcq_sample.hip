@@ -8,14 +8,14 @@
 //   epcq_idw_features    IDW weights + weighted features (epcq/epcq.py:434-452)
 //
 // Full-image sampling runs one 256-lane block per tile of rays.  The block
-// first stages its tile's halo region -- the hash entries of the
-// (tw + k - 1) x (th + k - 1) neighbour pixels and, row segment by row
+// first stages its tile's halo region -- the hash entries and bucket cones of
+// the (tw + k - 1) x (th + k - 1) neighbour pixels and, row segment by row
 // segment (contiguous in the row-major point lists), the coordinates and
 // indices of their points as float4 -- into LDS with coalesced loads; every
 // candidate walk of every ray then reads LDS.  A tile whose region holds more
-// points than the LDS capacity reads global memory instead.  Ray lists (no
-// tile structure) read global memory.  The per-ray logic is in
-// epcq_sampler.h.
+// points than the LDS capacity reads global memory instead, in the same
+// block.  Ray lists (no tile structure) read global memory.  The per-ray
+// logic is in epcq_sampler.h.
 #include <cmath>
 #include <mutex>
 
@@ -75,24 +75,9 @@ __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, Sampl
     __shared__ int s_total;
     const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total);
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const bool staged = sample_tile_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm);
-    if (!staged && threadIdx.x == 0 && view.tail) view.tail[1 + atomicAdd(view.tail, 1)] = tile;
-}
-
-// The tiles sample_tile_kernel skipped (halo region above the LDS capacity),
-// sampled from global memory.
-template <int MODE, int KMAX, int SMAX, int LAY>
-__global__ __launch_bounds__(kBlock) void sample_tile_fallback_kernel(SampleOpts o, SampleOut out, ViewArgs view,
-                                                                      TilePlan tp, int tiles_x) {
-    __shared__ int s_sum;
-    int tile;
-    if (view.tail) {  // the tiles sample_tile_kernel listed
-        if ((int)blockIdx.x >= view.tail[0]) return;
-        tile = view.tail[1 + blockIdx.x];
-    } else {
-        tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    }
-    sample_tile_fallback_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, &s_sum);
+    // a tile whose halo region exceeds the LDS reads global memory
+    if (!sample_tile_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm))
+        sample_tile_global<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0);
 }
 
 template <int MODE, int KMAX, int SMAX, int LAY>
@@ -115,9 +100,8 @@ __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restr
                                                              const float* __restrict__ radius, float r0,
                                                              const float* __restrict__ c2w, int64_t P,
                                                              float4* __restrict__ cone_a, float2* __restrict__ cone_b,
-                                                             float4* __restrict__ pts, int* __restrict__ tail) {
+                                                             float4* __restrict__ pts) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0 && tail) tail[0] = 0;  // the samplers' fallback list starts empty
     if (p < P) bucket_cone_body(hash, sorted_idx, xyz, nullptr, radius, r0, c2w, p, cone_a, cone_b, pts);
 }
 
@@ -239,18 +223,15 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
 
 ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash) {
     return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
-                    reinterpret_cast<const int2*>(hash), sorted_idx, nullptr, nullptr, nullptr, nullptr, nullptr};
+                    reinterpret_cast<const int2*>(hash), sorted_idx, nullptr, nullptr, nullptr, nullptr};
 }
 
 // Workspace layout of the samplers: cone_a [P] float4 | cone_b [P] float2 |
-// fallback tile list [P + 1] int | pixel-ordered points [n] float4.
-size_t cone_bytes(int64_t P) {
-    return align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256) + align_up((size_t)(P + 1) * 4, 256);
-}
+// pixel-ordered points [n] float4.
+size_t cone_bytes(int64_t P) { return align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256); }
 
-// Bucket cones, the fallback tile list and the pixel-ordered point copy in
-// the workspace (when it is large enough: then o.cull is set and view reads
-// them).
+// Bucket cones and the pixel-ordered point copy in the workspace (when it is
+// large enough: then o.cull is set and view reads them).
 void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera* cam, const int* sorted_idx,
                     const int* hash, void* workspace, size_t workspace_bytes, hipStream_t st) {
     const int64_t P = (int64_t)cam->width * cam->height;
@@ -259,16 +240,14 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     char* w = static_cast<char*>(workspace);
     float4* ca = reinterpret_cast<float4*>(w);
     float2* cb = reinterpret_cast<float2*>(w + align_up((size_t)P * 16, 256));
-    int* tail = reinterpret_cast<int*>(w + align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256));
     float4* pts = workspace_bytes >= cone_bytes(P) + (size_t)n * 16 ? reinterpret_cast<float4*>(w + cone_bytes(P))
                                                                      : nullptr;
     bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
-        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts, tail);
+        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts);
     o.cull = 1;
     view.cone_a = ca;
     view.cone_b = cb;
     view.pts = pts;
-    view.tail = tail;
 }
 
 template <int MODE, int KMAX, int SMAX, int LAY>
@@ -283,7 +262,6 @@ void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
         const dim3 block(tp.tw * tp.th);
         sample_tile_kernel<MODE, KMAX, SMAX, LAY><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp,
                                                                                           tiles_x);
-        sample_tile_fallback_kernel<MODE, KMAX, SMAX, LAY><<<dim3(tiles), block, 0, st>>>(o, out, view, tp, tiles_x);
     }
 }
 

@@ -787,7 +787,6 @@ struct ViewArgs {
     const float2* cone_b;
     const float4* pts;     // pixel-ordered points (bucket_cone_body) or nullptr
     const float* xyz;      // the view's cloud (nullptr: SampleOpts::xyz)
-    int* tail;             // [0] count, [1..] tiles left to the fallback pass; or nullptr
 };
 
 __device__ __forceinline__ void view_ray(const ViewArgs& v, int u, int vv, Ray& ray) {
@@ -834,32 +833,6 @@ __device__ __forceinline__ void tile_region(int W, int H, int k, int ko, int u0,
     const int vz = min(H - 1, v0 + th - 1 + (k - ko));
     RW = rz - ra + 1;
     RH = vz - va + 1;
-}
-
-// Number of points stage_tile would copy for this tile (block-wide, reads
-// the hash entries from global memory; every thread gets the result).
-__device__ __forceinline__ int region_points(const int2* __restrict__ hash, int W, int H, int k, int ko, int u0,
-                                             int v0, int tw, int th, int* s_sum) {
-    int ra, va, RW, RH;
-    tile_region(W, H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
-    if (threadIdx.x == 0) *s_sum = 0;
-    __syncthreads();
-    for (int r = threadIdx.x; r < RH; r += blockDim.x) {
-        int lo = 0, hi = 0;
-        bool found = false;
-        const int2* row = hash + (int64_t)(va + r) * W + ra;
-        for (int c = 0; c < RW; ++c) {
-            const int2 e = row[c];
-            if (e.y > 0) {
-                if (!found) lo = e.x;
-                found = true;
-                hi = e.x + e.y;
-            }
-        }
-        atomicAdd(s_sum, hi - lo);
-    }
-    __syncthreads();
-    return *s_sum;
 }
 
 // Stages the halo region of the tile with top-left ray (u0, v0) into LDS:
@@ -942,8 +915,8 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
 // One tile of one view: stage its halo region and sample its rays.  Output
 // row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
 // persistent kernels can call it in a loop.
-// Returns false (block-uniform) when the tile's region did not fit the LDS
-// and the tile is left to the fallback pass.
+// Returns false (block-uniform) when the tile's region did not fit the LDS;
+// the caller then samples the tile from global memory (sample_tile_global).
 template <int MODE, int KMAX, int SMAX, int LAY>
 __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
@@ -982,28 +955,6 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
         run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
                                        true);
     }
-}
-
-// A tile whose halo region exceeded the LDS capacity, from global memory.
-template <int MODE, int KMAX, int SMAX, int LAY>
-__device__ __forceinline__ void sample_tile_fallback_body(const SampleOpts& o0, const SampleOut& out,
-                                                          const ViewArgs& view, const TilePlan& tp, int tile,
-                                                          int tiles_x, int64_t ray_off, int* s_sum) {
-    SampleOpts o = o0;
-    if (view.xyz) o.xyz = view.xyz;
-    const int u0 = (tile % tiles_x) * tp.tw;
-    const int v0 = (tile / tiles_x) * tp.th;
-    const int total = region_points(view.hash, view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, s_sum);
-    const int u = u0 + (int)threadIdx.x % tp.tw;
-    const int v = v0 + (int)threadIdx.x / tp.tw;
-    if (total > tp.cap && u < view.W && v < view.H) {
-        Ray ray;
-        view_ray(view, u, v, ray);
-        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
-        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
-                                       true);
-    }
-    __syncthreads();
 }
 
 // Cone of pixel p's points seen from the view origin o (see cone_gap).  The
