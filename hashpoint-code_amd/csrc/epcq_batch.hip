@@ -11,7 +11,7 @@
 // so a graph captured once replays any view set that fits the arena.
 //
 // Pipeline (all views at once, one launch per stage):
-//   reset -> count -> col_partial -> bin_scan -> col_apply -> bin -> fine
+//   reset -> count -> col_partial -> bin_total -> bin_scan -> col_apply -> bin -> fine
 //   (the fused hash-table build of epcq_build.hip, per (view, chunk) / (view,
 //   bin) item) -> bounds (per-pixel bucket cones, pixel-ordered points) ->
 //   sample (one (view, tile) item at a time from an atomic work counter,
@@ -34,14 +34,13 @@ extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
 namespace epcq {
 
 float squared_threshold(float r);
-TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags);
 
 namespace {
 
 constexpr int kMaxBins = 1 << kMaxBinBits;      // LDS cursors of count / bin
 constexpr int kMaxFineTile = 1 << 12;          // pixels per bin (fine LDS)
 constexpr int kStages = 6;                     // item spaces: count, col, scan, fine, pixels, tiles
-enum { kCount = 0, kCol = 1, kScan = 2, kFine = 3, kPix = 4, kTile = 5 };
+enum { kCount = 0, kCol = 1, kScan = 2, kFine = 3, kTot = 4, kTile = 5 };
 
 struct BatchView {
     BuildView b;
@@ -100,6 +99,16 @@ __global__ __launch_bounds__(kBuildBlock) void batch_colpart_kernel(const BatchH
     }
 }
 
+__global__ __launch_bounds__(kBuildBlock) void batch_bintotal_kernel(const BatchHeader* __restrict__ h,
+                                                                     const BatchView* __restrict__ views,
+                                                                     const int* __restrict__ prefix) {
+    const int n_items = h->items[kTot], V = h->V;
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int v = find_view(prefix, V, item);
+        bin_total_body(views[v].b, (item - prefix[v]) * kBuildBlock + threadIdx.x);
+    }
+}
+
 __global__ __launch_bounds__(kBuildBlock) void batch_binscan_kernel(const BatchHeader* __restrict__ h,
                                                                     const BatchView* __restrict__ views) {
     __shared__ int s_wave[kBuildWaves];
@@ -136,10 +145,12 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         const int bs = bv.b.bstart[b], tot = bv.b.btotal[b];
         float4* pts = const_cast<float4*>(bv.view.pts);
         const float* xyz = bv.b.xyz;
+        float* prad = const_cast<float*>(bv.view.prad);
         for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock) {
             const int id = bv.b.out_idx[i];
             const float* q = xyz + 3 * (int64_t)id;
             pts[i] = make_float4(q[0], q[1], q[2], __int_as_float(id));
+            if (radius) prad[i] = radius[id];
         }
         __syncthreads();
         const int64_t p0 = (int64_t)b << bv.b.ts;
@@ -147,7 +158,8 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
             const int64_t p = p0 + f;
             if (p < bv.b.P)
                 bucket_cone_body(bv.view.hash, nullptr, nullptr, pts, radius, r0, bv.view.c2w, p,
-                                 const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b), nullptr);
+                                 const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b), nullptr,
+                                 nullptr);
         }
         __syncthreads();
     }
@@ -230,7 +242,7 @@ size_t header_bytes(int max_views) {
 struct ViewLayout {
     BuildPlan bp;
     int64_t P;
-    size_t ws, idx, hash, ca, cb, pts;  // byte offsets relative to the view's block
+    size_t ws, idx, hash, ca, cb, pts, prad;  // byte offsets relative to the view's block
     size_t bytes;
 };
 
@@ -250,6 +262,7 @@ ViewLayout layout_view(const epcq_view_desc& d) {
     L.ca = take((size_t)L.P * 16);
     L.cb = take((size_t)L.P * 8);
     L.pts = take((size_t)d.n * 16);
+    L.prad = take((size_t)d.n * 4);  // pixel-ordered radii (used with per-point radius)
     L.bytes = off;
     return L;
 }
@@ -302,6 +315,7 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     batch_reset_kernel<<<1, 1, 0, st>>>(h);
     batch_count_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 0);
+    batch_bintotal_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kTot * M);
     batch_binscan_kernel<<<gb, blk, 0, st>>>(h, dv);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 1);
     batch_bin_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
@@ -388,7 +402,7 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     epcq_tuning tu{16, 16, 3000, 0};
     if (tuning) tu = *tuning;
     else epcq_get_tuning(&tu);
-    b->tp = plan_tiles(opt->kernel_size, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags);
+    b->tp = plan_tiles(opt->kernel_size, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags, radius != nullptr);
     b->tp.xcd = 0;
     b->prefilter = (tu.flags & EPCQ_TUNE_PREFILTER) != 0;
     b->max_views = max_views;
@@ -489,7 +503,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         bv.view = ViewArgs{d_c2w, d.fx, d.fy, d.cx, d.cy, d.width, d.height, reinterpret_cast<const int2*>(out_hash),
                            out_idx, reinterpret_cast<const float4*>(vb + L[i].ca),
                            reinterpret_cast<const float2*>(vb + L[i].cb), reinterpret_cast<const float4*>(vb + L[i].pts),
-                           d.xyz};
+                           b->radius ? reinterpret_cast<const float*>(vb + L[i].prad) : nullptr, d.xyz};
         bv.tiles_x = (int)ceil_div(d.width, b->tp.tw);
         bv.tiles = bv.tiles_x * (int)ceil_div(d.height, b->tp.th);
         bv.col_groups = (int)ceil_div(L[i].bp.nbins, kBuildBlock);
@@ -505,7 +519,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         P[kCol * M + r + 1] = P[kCol * M + r] + bv.col_groups * bv.b.nseg;
         P[kScan * M + r + 1] = P[kScan * M + r] + 1;
         P[kFine * M + r + 1] = P[kFine * M + r] + bv.b.nbins;
-        P[kPix * M + r + 1] = P[kPix * M + r] + (int)ceil_div(bv.b.P, kBuildBlock);
+        P[kTot * M + r + 1] = P[kTot * M + r] + bv.col_groups;
         P[kTile * M + r + 1] = P[kTile * M + r] + bv.tiles;
     }
     b->header = BatchHeader{};

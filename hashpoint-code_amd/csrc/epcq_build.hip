@@ -11,9 +11,10 @@
 //   1. count    each 256-thread block projects a chunk of points and builds an
 //               LDS histogram over coarse bins (runs of 2^ts consecutive
 //               row-major pixels);
-//   2. scan     per-bin column scan of the block histograms (three small
-//               kernels) gives every (block, bin) a write cursor and every
-//               bin its start in the output;
+//   2. scan     per-bin column scan of the block histograms (four small
+//               kernels: segment sums, bin totals, bin starts, cursors)
+//               gives every (block, bin) a write cursor and every bin its
+//               start in the output;
 //   3. bin      blocks re-project their chunk and append each point to its
 //               bin; lanes with equal bins are matched with ballots and waves
 //               take their turn in order, so each bin receives its points in
@@ -38,6 +39,10 @@ __global__ __launch_bounds__(kBlock) void build_count_kernel(BuildView bv) {
 
 __global__ __launch_bounds__(kBlock) void col_partial_kernel(BuildView bv) {
     col_partial_body(bv, blockIdx.x * kBlock + threadIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(kBlock) void bin_total_kernel(BuildView bv) {
+    bin_total_body(bv, blockIdx.x * kBlock + threadIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BuildView bv) {
@@ -226,6 +231,7 @@ extern "C" int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2
     const dim3 cgrid((unsigned)ceil_div(bp.nbins, kBlock), (unsigned)bp.nseg);
     col_partial_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);
     if (int e = check_launch("build_hashtable/col_partial")) return e;
+    bin_total_kernel<<<dim3(cgrid.x), dim3(kBlock), 0, st>>>(bv);
     bin_scan_kernel<<<dim3(1), dim3(kBlock), 0, st>>>(bv);
     if (int e = check_launch("build_hashtable/bin_scan")) return e;
     col_apply_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);

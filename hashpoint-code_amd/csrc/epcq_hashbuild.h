@@ -275,31 +275,32 @@ __device__ __forceinline__ void col_partial_body(const BuildView& bv, int b, int
     bv.part[(int64_t)seg * bv.nbins + b] = s;
 }
 
-// One block: bin totals, bin starts (exclusive scan over bins, N_in at the
-// end) and absolute segment offsets part[seg][b].
+// Bin b's total over the segments; part[seg][b] becomes the exclusive
+// prefix over the segments (one thread per bin).
+__device__ __forceinline__ void bin_total_body(const BuildView& bv, int b) {
+    if (b >= bv.nbins) return;
+    int run = 0;
+    for (int seg = 0; seg < bv.nseg; ++seg) {
+        const int v = bv.part[(int64_t)seg * bv.nbins + b];
+        bv.part[(int64_t)seg * bv.nbins + b] = run;
+        run += v;
+    }
+    bv.btotal[b] = run;
+}
+
+// One block: bin starts, the exclusive scan of the bin totals (N_in at the
+// end).
 __device__ __forceinline__ void bin_scan_body(const BuildView& bv, int* s_wave, int* s_carry) {
-    const int nbins = bv.nbins, nseg = bv.nseg;
+    const int nbins = bv.nbins;
     if (threadIdx.x == 0) *s_carry = 0;
+    __syncthreads();
     for (int b0 = 0; b0 < nbins; b0 += kBuildBlock) {
         const int b = b0 + threadIdx.x;
-        int tot = 0;
-        if (b < nbins) {
-            for (int seg = 0; seg < nseg; ++seg) {
-                const int v = bv.part[(int64_t)seg * nbins + b];
-                bv.part[(int64_t)seg * nbins + b] = tot;
-                tot += v;
-            }
-        }
-        __syncthreads();
+        const int tot = b < nbins ? bv.btotal[b] : 0;
         const int carry = *s_carry;
         int total;
         const int ex = block_exclusive_scan(tot, s_wave, total);
-        if (b < nbins) {
-            const int start = carry + ex;
-            bv.bstart[b] = start;
-            bv.btotal[b] = tot;
-            for (int seg = 0; seg < nseg; ++seg) bv.part[(int64_t)seg * nbins + b] += start;
-        }
+        if (b < nbins) bv.bstart[b] = carry + ex;
         __syncthreads();
         if (threadIdx.x == 0) *s_carry = carry + total;
         __syncthreads();
@@ -312,7 +313,7 @@ __device__ __forceinline__ void bin_scan_body(const BuildView& bv, int* s_wave, 
 __device__ __forceinline__ void col_apply_body(const BuildView& bv, int b, int seg) {
     if (b >= bv.nbins) return;
     const int b0 = seg * kSegBlocks, b1 = min(b0 + kSegBlocks, bv.nblk);
-    int run = bv.part[(int64_t)seg * bv.nbins + b];
+    int run = bv.bstart[b] + bv.part[(int64_t)seg * bv.nbins + b];
     for (int blk = b0; blk < b1; ++blk) {
         const int64_t at = (int64_t)blk * bv.nbins + b;
         const int v = bv.hist[at];

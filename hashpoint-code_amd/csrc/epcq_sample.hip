@@ -38,20 +38,22 @@ float squared_threshold(float r) {
     return s;
 }
 
-TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags) {
+TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags, bool point_radius) {
     TilePlan p{};
     p.tw = tile_w;
     p.th = tile_h;
     p.xcd = (flags & EPCQ_TUNE_XCD_ORDER) != 0;
     p.rw_max = p.tw + k - 1;
     p.rh_max = p.th + k - 1;
+    // per point: float4 + radius; per region pixel: cone_a, hash, cone_b; per row: lo, base
     const size_t fixed =
         align_up((size_t)p.rw_max * p.rh_max * (16 + 8 + 8) + (size_t)(2 * p.rh_max + 1) * 4, 16);
     const size_t limit = 64 * 1024;
     size_t c = (size_t)std::max(cap, 0);
-    if (fixed + c * 16 > limit) c = fixed < limit ? (limit - fixed) / 16 : 0;
+    const size_t per = point_radius ? 20 : 16;
+    if (fixed + c * per > limit) c = fixed < limit ? (limit - fixed) / per : 0;
     p.cap = (int)c;
-    p.lds_bytes = c * 16 + fixed;
+    p.lds_bytes = c * per + fixed;
     return p;
 }
 
@@ -90,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
     Ray ray;
     view_ray(view, u, v, ray);
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a, view.cone_b,
+                        view.W};
     run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
 }
 
@@ -100,9 +103,9 @@ __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restr
                                                              const float* __restrict__ radius, float r0,
                                                              const float* __restrict__ c2w, int64_t P,
                                                              float4* __restrict__ cone_a, float2* __restrict__ cone_b,
-                                                             float4* __restrict__ pts) {
+                                                             float4* __restrict__ pts, float* __restrict__ prad) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, nullptr, radius, r0, c2w, p, cone_a, cone_b, pts);
+    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, nullptr, radius, r0, c2w, p, cone_a, cone_b, pts, prad);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each
@@ -223,11 +226,11 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
 
 ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* hash) {
     return ViewArgs{cam->c2w, cam->fx, cam->fy, cam->cx, cam->cy, cam->width, cam->height,
-                    reinterpret_cast<const int2*>(hash), sorted_idx, nullptr, nullptr, nullptr, nullptr};
+                    reinterpret_cast<const int2*>(hash), sorted_idx, nullptr, nullptr, nullptr, nullptr, nullptr};
 }
 
 // Workspace layout of the samplers: cone_a [P] float4 | cone_b [P] float2 |
-// pixel-ordered points [n] float4.
+// pixel-ordered points [n] float4 | pixel-ordered radii [n] float.
 size_t cone_bytes(int64_t P) { return align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256); }
 
 // Bucket cones and the pixel-ordered point copy in the workspace (when it is
@@ -240,14 +243,17 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     char* w = static_cast<char*>(workspace);
     float4* ca = reinterpret_cast<float4*>(w);
     float2* cb = reinterpret_cast<float2*>(w + align_up((size_t)P * 16, 256));
-    float4* pts = workspace_bytes >= cone_bytes(P) + (size_t)n * 16 ? reinterpret_cast<float4*>(w + cone_bytes(P))
-                                                                     : nullptr;
+    const size_t pts_off = cone_bytes(P), rad_off = pts_off + align_up((size_t)n * 16, 256);
+    float4* pts = workspace_bytes >= rad_off ? reinterpret_cast<float4*>(w + pts_off) : nullptr;
+    float* prad = o.radius && workspace_bytes >= rad_off + (size_t)n * 4 ? reinterpret_cast<float*>(w + rad_off)
+                                                                        : nullptr;
     bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
-        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts);
+        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts, prad);
     o.cull = 1;
     view.cone_a = ca;
     view.cone_b = cb;
     view.pts = pts;
+    view.prad = prad;
 }
 
 template <int MODE, int KMAX, int SMAX, int LAY>
@@ -285,7 +291,7 @@ int launch_sample(const SampleOpts& o, const SampleOut& out, const ViewArgs& vie
                   int mode, int bound, hipStream_t st, const char* who) {
     if (R == 0) return EPCQ_OK;
     const epcq_tuning tu = current_tuning();
-    const TilePlan tp = plan_tiles(o.k, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags);
+    const TilePlan tp = plan_tiles(o.k, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags, o.radius != nullptr);
     if (mode == EPCQ_MODE_COMPAT) {
         launch_smax<EPCQ_MODE_COMPAT, kCompatBuffer>(o, out, view, rays, R, tp, bound, st);
     } else if (o.Kc <= 8) {
@@ -355,7 +361,7 @@ extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max) {
 
 extern "C" size_t epcq_sampling_workspace_size(int64_t n, int32_t height, int32_t width) {
     if (height <= 0 || width <= 0 || n < 0) return 0;
-    return cone_bytes((int64_t)height * width) + (size_t)n * 16;
+    return cone_bytes((int64_t)height * width) + align_up((size_t)n * 16, 256) + (size_t)n * 4;
 }
 
 extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,

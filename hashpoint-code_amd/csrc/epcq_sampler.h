@@ -74,7 +74,6 @@ struct SampleOut {
     float* agg;
 };
 
-// Per-pixel bucket cone (see bucket_cone_body): ca = (axis, P), cb = (Q, B).
 // Output layouts, fixed per kernel instantiation: the dense drop-in arrays,
 // the compact arrays, or the compact arrays with inverse-distance weighted
 // features.
@@ -82,11 +81,16 @@ constexpr int kLayDense = 0;
 constexpr int kLayCompact = 1;
 constexpr int kLayFeat = 2;
 
+// Candidate sources.  Per-pixel bucket cone (see bucket_cone_body): ca =
+// (axis, P), cb = (Q, B); point(j) = (x, y, z, index bits) of list position
+// j; rad(j, p) = its radius when radii are per point (SampleOpts::radius).
 struct SrcGlobal {
     const int2* hash;
     const int* sorted_idx;
     const float* xyz;
     const float4* pts;   // pixel-ordered (x, y, z, index bits) or nullptr (gather)
+    const float* prad;   // pixel-ordered radii or nullptr (gather)
+    const float* radius;  // per-point radii by index (when prad is null)
     const float4* cone_a;
     const float2* cone_b;
     int W;
@@ -99,14 +103,19 @@ struct SrcGlobal {
         const float* p = xyz + 3 * (int64_t)id;
         return make_float4(p[0], p[1], p[2], __int_as_float(id));
     }
+    __device__ __forceinline__ float rad(int j, const float4& p) const {
+        return prad ? prad[j] : radius[__float_as_int(p.w)];
+    }
 };
 
 struct SrcLds {
     const int2* hash;
     const float4* pts;
+    const float* prad;  // staged radii (per-point radius only)
     const float4* cone_a;
     const float2* cone_b;
     int ra, va, RW;
+    __device__ __forceinline__ float rad(int j, const float4&) const { return prad[j]; }
     __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[(vb - va) * RW + (ub - ra)]; }
     __device__ __forceinline__ float4 cone4(int ub, int vb) const { return cone_a[(vb - va) * RW + (ub - ra)]; }
     __device__ __forceinline__ float2 cone2(int ub, int vb) const { return cone_b[(vb - va) * RW + (ub - ra)]; }
@@ -168,9 +177,10 @@ __device__ __forceinline__ bool cone_clears(const Ray& ray, const float4& ca, co
     return sq * (ca.w * ca.w) > x * x;
 }
 
-// dist < radius for a squared distance, exact (see the file comment).
-__device__ __forceinline__ bool within(const SampleOpts& o, float e2, int id) {
-    return o.radius ? sqrtf(e2) < o.radius[id] : e2 < o.r2lim;
+// dist < radius for a squared distance, exact (see the file comment); r is
+// the point's radius when radii are per point.
+__device__ __forceinline__ bool within(const SampleOpts& o, float e2, float r) {
+    return o.radius ? sqrtf(e2) < r : e2 < o.r2lim;
 }
 
 // Lane sets of one ray: bit l for window lane l (k*k lanes, at most 64).
@@ -275,13 +285,12 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
-            const int id = __float_as_int(p.w);
-            const float r = o.radius ? o.radius[id] : o.r0;
+            const float r = o.radius ? src.rad(e.x + j, p) : o.r0;
             if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))
                 continue;
             float t, e2;
             ray_t_e2(ray, p, t, e2);
-            if (within(o, e2, id)) {
+            if (within(o, e2, r)) {
                 ++kept;
                 fn(t, e2);
             }
@@ -409,13 +418,14 @@ template <int MODE, int KMAX, int NS, class Src>
 __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
                                                const float (&sp)[NS][3], int ns, LaneMask mask,
                                                NbBuf<MODE, KMAX> (&buf)[NS]) {
-    auto visit = [&](const float4& p) {
+    auto visit = [&](const float4& p, int jp) {
         const int id = __float_as_int(p.w);
+        const float r = o.radius ? src.rad(jp, p) : o.r0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             if (s < ns) {
                 const float d2 = sqnorm3(sp[s][0] - p.x, sp[s][1] - p.y, sp[s][2] - p.z);
-                if (within(o, d2, id)) buf[s].add(sqrtf(d2), id);
+                if (within(o, d2, r)) buf[s].add(sqrtf(d2), id);
             }
         }
     };
@@ -426,7 +436,7 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
         for (int j = 0; j < e.y; ++j) {
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
-            visit(p);
+            visit(p, e.x + j);
         }
     };
     if (MODE == EPCQ_MODE_COMPAT || o.L <= 64) {
@@ -786,6 +796,7 @@ struct ViewArgs {
     const float4* cone_a;  // bucket cones (bucket_cone_body) or nullptr
     const float2* cone_b;
     const float4* pts;     // pixel-ordered points (bucket_cone_body) or nullptr
+    const float* prad;     // pixel-ordered radii (per-point radius) or nullptr
     const float* xyz;      // the view's cloud (nullptr: SampleOpts::xyz)
 };
 
@@ -805,6 +816,7 @@ struct TileSmem {
     float2* cb;
     int* rowlo;
     int* rowbase;
+    float* rad;  // [cap] radii of the staged points (per-point radius)
     int* total;
 };
 
@@ -817,11 +829,13 @@ __device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan&
     m.cb = reinterpret_cast<float2*>(m.hash + rr);
     m.rowlo = reinterpret_cast<int*>(m.cb + rr);
     m.rowbase = m.rowlo + tp.rh_max;
+    m.rad = reinterpret_cast<float*>(m.rowbase + tp.rh_max + 1);
     m.total = total;
     return m;
 }
 
-TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags);
+// LDS plan of the tiled samplers; radii are staged only with per-point radius.
+TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags, bool point_radius);
 
 // Halo region of the tile with top-left ray (u0, v0): pixels [ra, ra+RW) x
 // [va, va+RH) of the image.
@@ -840,9 +854,9 @@ __device__ __forceinline__ void tile_region(int W, int H, int k, int ko, int u0,
 // culling) and points.  Returns the number of staged points, or -1 if the
 // region holds more than `cap` points (the tile then reads global memory).
 // Block-wide; every thread of the block must call it.
-__device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __restrict__ xyz, bool cull, int k,
-                                          int ko, int u0, int v0, int tw, int th, int cap, const TileSmem& sm,
-                                          int& ra, int& va, int& RW) {
+__device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __restrict__ xyz,
+                                          const float* __restrict__ radius, bool cull, int k, int ko, int u0, int v0,
+                                          int tw, int th, int cap, const TileSmem& sm, int& ra, int& va, int& RW) {
     int RH;
     const int W = view.W;
     tile_region(W, view.H, k, ko, u0, v0, tw, th, ra, va, RW, RH);
@@ -893,13 +907,17 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
             else hi = mid;
         }
         const int gi = sm.rowlo[lo] + (i - sm.rowbase[lo]);
+        int id;
         if (view.pts) {
-            sm.pts[i] = view.pts[gi];
+            const float4 q = view.pts[gi];
+            sm.pts[i] = q;
+            id = __float_as_int(q.w);
         } else {
-            const int id = view.sorted_idx[gi];
+            id = view.sorted_idx[gi];
             const float* p = xyz + 3 * (int64_t)id;
             sm.pts[i] = make_float4(p[0], p[1], p[2], __int_as_float(id));
         }
+        if (radius) sm.rad[i] = view.prad ? view.prad[gi] : radius[id];
     }
     for (int i = tid; i < RW * RH; i += nthr) {
         const int2 e = sm.hash[i];
@@ -926,13 +944,14 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     const int u0 = (tile % tiles_x) * tp.tw;
     const int v0 = (tile / tiles_x) * tp.th;
     int ra, va, RW;
-    const int staged = stage_tile(view, o.xyz, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va, RW);
+    const int staged =
+        stage_tile(view, o.xyz, o.radius, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va, RW);
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
     if (staged >= 0 && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcLds src{sm.hash, sm.pts, sm.ca, sm.cb, ra, va, RW};
+        const SrcLds src{sm.hash, sm.pts, sm.rad, sm.ca, sm.cb, ra, va, RW};
         run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
                                        true);
     }
@@ -951,7 +970,8 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
     if (u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.cone_a, view.cone_b, view.W};
+        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
+                            view.cone_b, view.W};
         run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
                                        true);
     }
@@ -960,7 +980,8 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
 // Cone of pixel p's points seen from the view origin o (see cone_gap).  The
 // points come from pts_in (pixel-ordered (x, y, z, index bits)) when given,
 // else through sorted_idx / xyz, in which case they are also copied in list
-// order into pts (when not null).  One thread per pixel; two passes over the
+// order into pts (when not null); with per-point radii, the radii are copied
+// in list order into prad (when not null).  One thread per pixel; two passes over the
 // bucket: the axis (the normalised sum of the unit directions, each oriented
 // to the first one's side, since cones bound lines), then the largest angle
 // to it.  A bucket holding a point at o or a non-finite coordinate gets a cone
@@ -970,7 +991,7 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
                                                  const float* __restrict__ radius, float r0,
                                                  const float* __restrict__ c2w, int64_t p,
                                                  float4* __restrict__ cone_a, float2* __restrict__ cone_b,
-                                                 float4* __restrict__ pts) {
+                                                 float4* __restrict__ pts, float* __restrict__ prad) {
     const int2 e = hash[p];
     const float never = __builtin_inff();
     if (e.y <= 0) {
@@ -992,7 +1013,11 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
         const float4 q = point(j);
         const int id = __float_as_int(q.w);
         if (!pts_in && pts) pts[e.x + j] = q;
-        if (radius) rb = j == 0 ? radius[id] : fmaxf(rb, radius[id]);
+        if (radius) {
+            const float rj = radius[id];
+            if (prad) prad[e.x + j] = rj;
+            rb = j == 0 ? rj : fmaxf(rb, rj);
+        }
         const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
         const float n = sqrtf(sqnorm3(w0, w1, w2));
         if (!(n > 0.0f && n < never)) {
