@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch shape, no autotuning")
     ap.add_argument("--no-serve", action="store_true", help="batched: skip the serving sweep")
-    ap.add_argument("--tuning", default=None, help="batched: launch shape tile_w,tile_h,tile_cap,flags (no autotuning)")
+    ap.add_argument("--tuning", default=None, help="batched: launch shape tile_w,tile_h,tile_cap,flags[,heavy] (no autotuning)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     ap.add_argument("--plain-passes", type=int, default=3, help="batched: timed passes of the per-view forms")
@@ -128,15 +128,16 @@ def bench_batched(args, wl):
     arena = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     tune_log = {}
     if args.tuning:
-        tw, th, cap, fl = (int(x) for x in args.tuning.split(","))
-        best = {"tile_w": tw, "tile_h": th, "tile_cap": cap, "flags": fl}
+        vals = [int(x) for x in args.tuning.split(",")]
+        best = dict(autotune.DEFAULT)
+        best.update(zip(("tile_w", "tile_h", "tile_cap", "flags", "heavy"), vals))
     elif args.no_tune:
         best = dict(autotune.DEFAULT)
     else:
         cands = {}
 
         def run(t):
-            key = (t["tile_w"], t["tile_h"], t["tile_cap"], t["flags"])
+            key = (t["tile_w"], t["tile_h"], t["tile_cap"], t["flags"], t["heavy"])
             if key not in cands:
                 torch.cuda.synchronize()  # the previous candidate is idle before its arena is rewritten
                 s = MultiViewSampler(opt, mode=args.mode, device=dev, max_views=V, tuning=t)

@@ -173,7 +173,8 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
     extern __shared__ float4 s_dyn[];
     __shared__ int s_total;
     __shared__ int s_item;
-    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total);
+    __shared__ int s_heavy[kMaxTileRays + 1];
+    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
     const int n_items = h->items[kTile], V = h->V;
     const SampleOut out = h->out;
     while (true) {
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
                                                                             bv.tiles_x, bv.ray_off, sm);
         if (!staged)
             sample_tile_global<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
-                                                              bv.ray_off);
+                                                              bv.ray_off, sm);
     }
 }
 
@@ -206,6 +207,7 @@ struct epcq_batch {
     int mode, s_max, bound;
     TilePlan tp;
     int prefilter;
+    int heavy;
     int max_views;
     char* arena;
     size_t arena_bytes;
@@ -338,6 +340,7 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     o.xyz = nullptr;  // each view's cloud comes from its descriptor
     o.prefilter = b->prefilter;
     o.cull = 1;
+    o.heavy = b->heavy;
     if (b->mode == EPCQ_MODE_COMPAT) enqueue_sample_smax<EPCQ_MODE_COMPAT, kCompatBuffer>(b, o, st);
     else if (o.Kc <= 8) enqueue_sample_smax<EPCQ_MODE_KNN, 8>(b, o, st);
     else if (o.Kc <= 16) enqueue_sample_smax<EPCQ_MODE_KNN, 16>(b, o, st);
@@ -399,12 +402,13 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     b->mode = mode;
     b->s_max = s_max;
     b->bound = bound;
-    epcq_tuning tu{16, 16, 3000, 0};
+    epcq_tuning tu{16, 16, 3000, 0, kHeavyCandidates};
     if (tuning) tu = *tuning;
     else epcq_get_tuning(&tu);
     b->tp = plan_tiles(opt->kernel_size, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags, radius != nullptr);
     b->tp.xcd = 0;
     b->prefilter = (tu.flags & EPCQ_TUNE_PREFILTER) != 0;
+    b->heavy = std::max(tu.heavy, 0);
     b->max_views = max_views;
     b->arena = static_cast<char*>(arena);
     b->arena_bytes = arena_bytes;

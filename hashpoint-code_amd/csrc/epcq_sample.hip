@@ -63,7 +63,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxLiveSamples = 16;  // samples per ray the sampler can emit
 
 std::mutex g_tuning_mu;
-epcq_tuning g_tuning{16, 16, 3000, 0};
+epcq_tuning g_tuning{16, 16, 3000, 0, kHeavyCandidates};
 
 epcq_tuning current_tuning() {
     std::lock_guard<std::mutex> lk(g_tuning_mu);
@@ -75,26 +75,46 @@ __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, Sampl
                                                              int tiles_x) {
     extern __shared__ float4 s_dyn[];
     __shared__ int s_total;
-    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total);
+    __shared__ int s_heavy[kMaxTileRays + 1];
+    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     // a tile whose halo region exceeds the LDS reads global memory
     if (!sample_tile_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm))
-        sample_tile_global<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0);
+        sample_tile_global<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm);
 }
 
+// One thread per listed ray; rays with many candidates are deferred and then
+// run one per wave (the block's waves share the queue).
 template <int MODE, int KMAX, int SMAX, int LAY>
 __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, SampleOut out, ViewArgs view,
                                                              const int* __restrict__ rays, int64_t R) {
-    const int64_t ri = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (ri >= R) return;
-    const int64_t q = rays[ri];
-    const bool valid = q >= 0 && q < (int64_t)view.W * view.H;
-    const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
-    Ray ray;
-    view_ray(view, u, v, ray);
+    __shared__ int s_heavy[kMaxTileRays + 1];
+    static_assert(kBlock <= kMaxTileRays, "queue holds one entry per thread");
+    if (threadIdx.x == 0) s_heavy[kMaxTileRays] = 0;
+    __syncthreads();
     const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a, view.cone_b,
                         view.W};
-    run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ri, valid);
+    const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t ri = r0 + threadIdx.x;
+    if (ri < R) {
+        const int64_t q = rays[ri];
+        const bool valid = q >= 0 && q < (int64_t)view.W * view.H;
+        const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
+        Ray ray;
+        view_ray(view, u, v, ray);
+        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ri, valid, true))
+            defer_ray(s_heavy, (int)threadIdx.x);
+    }
+    __syncthreads();
+    const int n = s_heavy[kMaxTileRays];
+    for (int h = (int)threadIdx.x / kWave; h < n; h += kBlock / kWave) {
+        const int64_t rj = r0 + s_heavy[h];
+        const int64_t q = rays[rj];  // deferred rays are valid
+        const int u = (int)(q % view.W), v = (int)(q / view.W);
+        Ray ray;
+        view_ray(view, u, v, ray);
+        run_ray_wave<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, rj);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restrict__ hash,
@@ -220,7 +240,9 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     o.S = opt->num_sample_points_per_ray;
     o.Kc = opt->num_point_cloud_per_sp;
     o.xyz = xyz;
-    o.prefilter = (current_tuning().flags & EPCQ_TUNE_PREFILTER) != 0;
+    const epcq_tuning tu = current_tuning();
+    o.prefilter = (tu.flags & EPCQ_TUNE_PREFILTER) != 0;
+    o.heavy = tu.heavy;
     return o;
 }
 
@@ -325,6 +347,7 @@ extern "C" int epcq_set_tuning(const epcq_tuning* t) {
         return fail(EPCQ_ERR_INVALID,
                     "set_tuning: tile_w must be a power of two <= 64 and tile_w * tile_h 64, 128 or 256");
     if (t->tile_cap < 0) return fail(EPCQ_ERR_INVALID, "set_tuning: negative tile_cap");
+    if (t->heavy < 0) return fail(EPCQ_ERR_INVALID, "set_tuning: negative heavy");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning = *t;
     return EPCQ_OK;

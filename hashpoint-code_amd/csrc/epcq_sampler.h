@@ -42,6 +42,8 @@
 namespace epcq {
 
 constexpr int kSlotTable = 8;
+// Rays whose hit lanes hold more candidates than this are run by a whole wave.
+constexpr int kHeavyCandidates = 128;
 
 struct SampleOpts {
     int k, ko, L;         // window size, ceil(k/2), k*k
@@ -55,6 +57,7 @@ struct SampleOpts {
     int feat_bf16, F;
     int prefilter;        // cross-product rejection before the exact test
     int cull;             // skip neighbour buckets by their bounding spheres
+    int heavy;            // a ray whose hit lanes hold more candidates is run by its whole wave (0: never)
 };
 
 struct SampleOut {
@@ -224,22 +227,28 @@ __device__ __forceinline__ void for_each_masked_lane(const SampleOpts& o, int u,
 
 // Lanes whose bucket is non-empty (emask) and, of those, the ones whose cone
 // does not clear the ray by `margin` (returned; every lane when culling is
-// off).  Only the returned lanes can hold hits.
+// off).  Only the returned lanes can hold hits; `cand` counts their points
+// (0 for windows without masks).
 template <class Src>
 __device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                              int W, int H, float margin, LaneMask& emask) {
+                                              int W, int H, float margin, LaneMask& emask, int& cand) {
+    cand = 0;
     if (o.L > 64) {
         emask = ~0ull;
         return ~0ull;
     }
     LaneMask m = 0, e = 0;
+    int c = 0;
     for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
-        if (src.bucket(ub, vb).y <= 0) return;
+        const int n = src.bucket(ub, vb).y;
+        if (n <= 0) return;
         e |= 1ull << l;
         if (o.cull && cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) return;
         m |= 1ull << l;
+        c += n;
     });
     emask = e;
+    cand = c;
     return m;
 }
 
@@ -537,11 +546,64 @@ __device__ __forceinline__ void write_neighbours(const SampleOpts& o, const Samp
     }
 }
 
+// Writes emitted sample number ns of ray ri (depth ts along the ray).
+template <int LAY>
+__device__ __forceinline__ void write_sample(const SampleOpts& o, const SampleOut& out, const Ray& ray, int64_t ri,
+                                             int ns, float ts) {
+    if constexpr (LAY == kLayDense) {
+        const int64_t at = ri * o.S + ns;
+        out.sp_xyz[3 * at + 0] = ray.o[0] + ts * ray.d[0];
+        out.sp_xyz[3 * at + 1] = ray.o[1] + ts * ray.d[1];
+        out.sp_xyz[3 * at + 2] = ray.o[2] + ts * ray.d[2];
+        out.sp_t[at] = ts;
+        out.mask[at] = 1;
+    } else if (ns < out.s_max) {
+        out.sample_t[ri * out.s_max + ns] = ts;
+    }
+}
+
+// Every output element of ray ri not written for its ns emitted samples:
+// zeros / -1 / false, and the sample count.
+template <int MODE, int LAY>
+__device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut& out, int64_t ri, int ns) {
+    if constexpr (LAY == kLayDense) {
+        for (int s = ns; s < o.S; ++s) {
+            const int64_t at = ri * o.S + s;
+            out.sp_xyz[3 * at + 0] = 0.0f;
+            out.sp_xyz[3 * at + 1] = 0.0f;
+            out.sp_xyz[3 * at + 2] = 0.0f;
+            out.sp_t[at] = 0.0f;
+            out.mask[at] = 0;
+            int* dst = out.pc_idx + at * o.Kc;
+            for (int b = 0; b < o.Kc; ++b) dst[b] = -1;
+        }
+    } else {
+        out.n_samples[ri] = ns;
+        for (int s = ns; s < out.s_max; ++s) {
+            const int64_t at = ri * out.s_max + s;
+            out.sample_t[at] = 0.0f;
+            for (int b = 0; b < o.Kc; ++b) {
+                out.nb_idx[at * o.Kc + b] = -1;
+                if constexpr (MODE == EPCQ_MODE_KNN) {
+                    if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
+                    if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
+                }
+                if constexpr (LAY == kLayFeat) out.w[at * o.Kc + b] = 0.0f;
+            }
+            if constexpr (LAY == kLayFeat)
+                for (int f = 0; f < o.F; ++f) out.agg[at * o.F + f] = 0.0f;
+        }
+    }
+}
+
 // Samples and neighbours of one ray; writes every output element of row ri.
 // SMAX bounds the samples one ray can emit (epcq_max_live_samples).
+// With may_defer, a ray whose hit lanes hold more than o.heavy candidates is
+// left untouched and false returned: the caller then runs it with
+// run_ray_wave, which spreads its candidates over a whole wave.
 template <int MODE, int KMAX, int SMAX, int LAY, class Src>
-__device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
-                        int W, int H, int64_t ri, bool valid_ray) {
+__device__ bool run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
+                        int W, int H, int64_t ri, bool valid_ray, bool may_defer) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;  // sample points served by one neighbour walk
     constexpr bool dense = LAY == kLayDense;
     int ns = 0;
@@ -550,7 +612,9 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
     for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
     // lanes whose bucket may hold a hit (a ray with none is empty)
     LaneMask emask = 0;
-    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin, emask) : 0ull;
+    int cand = 0;
+    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin, emask, cand) : 0ull;
+    if (may_defer && o.heavy > 0 && cand > o.heavy) return false;
     if (valid_ray && hmask != 0ull) {
         // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
         float tmin = 1e3f, tmax = -1e3f;
@@ -678,19 +742,7 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
                 const float wgt = alpha * T;
                 if (wgt > 1e-3f) {
                     const float ts = tmin + (float)s * o.t_intvl;
-                    const float s0 = ray.o[0] + ts * ray.d[0];
-                    const float s1 = ray.o[1] + ts * ray.d[1];
-                    const float s2 = ray.o[2] + ts * ray.d[2];
-                    if constexpr (dense) {
-                        const int64_t at = ri * o.S + ns;
-                        out.sp_xyz[3 * at + 0] = s0;
-                        out.sp_xyz[3 * at + 1] = s1;
-                        out.sp_xyz[3 * at + 2] = s2;
-                        out.sp_t[at] = ts;
-                        out.mask[at] = 1;
-                    } else if (ns < out.s_max) {
-                        out.sample_t[ri * out.s_max + ns] = ts;
-                    }
+                    write_sample<LAY>(o, out, ray, ri, ns, ts);
 #pragma unroll
                     for (int q = 0; q < SMAX; ++q)
                         if (q == ns) ts_arr[q] = ts;
@@ -734,34 +786,253 @@ __device__ void run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
     }
 
     // Everything not written above: zeros / -1 / false.
-    if constexpr (dense) {
-        for (int s = ns; s < o.S; ++s) {
-            const int64_t at = ri * o.S + s;
-            out.sp_xyz[3 * at + 0] = 0.0f;
-            out.sp_xyz[3 * at + 1] = 0.0f;
-            out.sp_xyz[3 * at + 2] = 0.0f;
-            out.sp_t[at] = 0.0f;
-            out.mask[at] = 0;
-            int* dst = out.pc_idx + at * o.Kc;
-            for (int b = 0; b < o.Kc; ++b) dst[b] = -1;
-        }
-    } else {
-        out.n_samples[ri] = ns;
-        for (int s = ns; s < out.s_max; ++s) {
-            const int64_t at = ri * out.s_max + s;
-            out.sample_t[at] = 0.0f;
-            for (int b = 0; b < o.Kc; ++b) {
-                out.nb_idx[at * o.Kc + b] = -1;
-                if constexpr (MODE == EPCQ_MODE_KNN) {
-                    if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
-                    if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
+    write_tail<MODE, LAY>(o, out, ri, ns);
+    return true;
+}
+
+// ---- wave-cooperative form ---------------------------------------------------
+// run_ray_wave computes exactly what run_ray computes for one ray, with all 64
+// lanes of the wave working on it: lane l < L tests window lane l, and each
+// walk reads the candidates of a bucket 64 at a time, one per lane.  The
+// order-dependent steps take the candidates in walk order (set bits of a
+// ballot, lowest lane first):
+//   - the 48-hit cap of a bucket keeps the hits of rank < 48 among its hits;
+//   - slot sums: lane q accumulates slots q and q + 64, each in walk order;
+//   - neighbour buffers: every lane holds the same buffer state and adds the
+//     accepted candidates one at a time, in walk order.
+// tmin / tmax are plain min / max reductions (a +-0 tie cannot change any
+// output: t - tmin and tmin + s * t_intvl round the same either way).  Every
+// lane must call it with the same ray, pixel and row (wave-uniform control
+// flow); lane 0 writes the outputs.
+
+__device__ __forceinline__ float lane_bcast(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ int lane_bcast(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+__device__ __forceinline__ float wave_min(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float y = __shfl_xor(x, o);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float y = __shfl_xor(x, o);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+template <int MODE, int KMAX, int SMAX, int LAY, class Src>
+__device__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
+                             int W, int H, int64_t ri) {
+    constexpr int NS = KMAX <= 8 ? 2 : 1;
+    const int lane = __lane_id();
+    // window lane `lane`: its bucket and cone test (same margins as run_ray)
+    int2 e = make_int2(0, -1);
+    int ubl = 0, vbl = 0;
+    bool nonempty = false, maybe = false;
+    if (lane < o.L) {
+        const int r = (lane * o.kdiv) >> 16;
+        ubl = min(max(u + (lane - r * o.k) - (o.ko - 1), 0), W - 1);
+        vbl = min(max(v + r - (o.ko - 1), 0), H - 1);
+        e = src.bucket(ubl, vbl);
+        nonempty = e.y > 0;
+        maybe = nonempty && !(o.cull && cone_clears(ray, src.cone4(ubl, vbl), src.cone2(ubl, vbl), ray.omargin));
+    }
+    const LaneMask emask = __ballot(nonempty);
+    const LaneMask hmask = __ballot(maybe);
+
+    // fn(keep, t, e2) on every lane for each 64-candidate chunk of the hit
+    // walk, `keep` set on the lanes holding a hit that the 48 cap keeps
+    auto hit_walk = [&](auto&& fn) {
+        LaneMask m = hmask;
+        while (m) {
+            const int l = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1ull;
+            const int ex = lane_bcast(e.x, l), ey = lane_bcast(e.y, l);
+            int kept = 0;
+            for (int c0 = 0; c0 < ey && kept < kSampleLaneHits; c0 += kWave) {
+                const int j = c0 + lane;
+                bool hit = false;
+                float t = 0.0f, e2 = 0.0f;
+                if (j < ey) {
+                    const float4 p = src.point(ex + j);
+                    const float r = o.radius ? src.rad(ex + j, p) : o.r0;
+                    if (!(o.prefilter &&
+                          surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))) {
+                        ray_t_e2(ray, p, t, e2);
+                        hit = within(o, e2, r);
+                    }
                 }
-                if constexpr (LAY == kLayFeat) out.w[at * o.Kc + b] = 0.0f;
+                const uint64_t hb = __ballot(hit);
+                const bool keep = hit && kept + __popcll(hb & lanemask_lt()) < kSampleLaneHits;
+                fn(keep, t, e2);
+                kept += __popcll(hb);
             }
-            if constexpr (LAY == kLayFeat)
-                for (int f = 0; f < o.F; ++f) out.agg[at * o.F + f] = 0.0f;
+        }
+    };
+
+    int ns = 0;
+    float ts_arr[SMAX];
+#pragma unroll
+    for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
+    if (hmask != 0ull) {
+        // Step 1: tmin / tmax
+        float tmin = 1e3f, tmax = -1e3f;
+        hit_walk([&](bool keep, float t, float) {
+            if (keep) {
+                if (tmin > t) tmin = t;
+                if (tmax < t) tmax = t;
+            }
+        });
+        tmin = wave_min(tmin);
+        tmax = wave_max(tmax);
+        if (!(tmin > tmax)) {
+            // Step 2: slot sums in walk order; lane q owns slots q and q + 64
+            float sum0 = 0.0f, sum1 = 0.0f, cnt0 = 0.0f, cnt1 = 0.0f;
+            hit_walk([&](bool keep, float t, float e2) {
+                int sl = 0;
+                float dist = 0.0f;
+                bool ok = false;
+                if (keep) {
+                    const float qf = (t - tmin) / o.t_intvl;
+                    ok = qf > -1.0f && qf < (float)kMaxSlots;
+                    if (ok) {
+                        sl = (int)qf;
+                        dist = sqrtf(e2);
+                    }
+                }
+                uint64_t b = __ballot(ok);
+                while (b) {
+                    const int x = __ffsll((unsigned long long)b) - 1;
+                    b &= b - 1ull;
+                    const int sx = lane_bcast(sl, x);
+                    const float dx = lane_bcast(dist, x);
+                    if ((sx & (kWave - 1)) == lane) {
+                        if (sx < kWave) {
+                            sum0 = sum0 + dx;
+                            cnt0 = cnt0 + 1.0f;
+                        } else {
+                            sum1 = sum1 + dx;
+                            cnt1 = cnt1 + 1.0f;
+                        }
+                    }
+                }
+            });
+            // Step 3: front-to-back walk over the slots with >= 2 hits
+            const uint64_t two0 = __ballot(cnt0 >= 2.0f), two1 = __ballot(cnt1 >= 2.0f);
+            float T = 1.0f;
+            for (int half = 0; half < 2; ++half) {
+                uint64_t b = half == 0 ? two0 : two1;
+                bool stop = false;
+                while (b) {
+                    const int x = __ffsll((unsigned long long)b) - 1;
+                    b &= b - 1ull;
+                    const int sl = half * kWave + x;
+                    const float bsum = lane_bcast(half == 0 ? sum0 : sum1, x);
+                    const float bcnt = lane_bcast(half == 0 ? cnt0 : cnt1, x);
+                    if (T < 1e-3f || ns >= o.S) {
+                        stop = true;
+                        break;
+                    }
+                    const float sdf = bsum / (bcnt + 1e-6f);
+                    const float alpha = o.gamma * expf_cw(-sdf / o.alpha_p);
+                    const float wgt = alpha * T;
+                    if (wgt > 1e-3f) {
+                        const float ts = tmin + (float)sl * o.t_intvl;
+                        if (lane == 0) write_sample<LAY>(o, out, ray, ri, ns, ts);
+#pragma unroll
+                        for (int q = 0; q < SMAX; ++q)
+                            if (q == ns) ts_arr[q] = ts;
+                        ++ns;
+                    }
+                    T -= alpha;
+                }
+                if (stop) break;
+            }
+        }
+        // Step 4: neighbours of the emitted samples, NS per walk
+        constexpr bool dense = LAY == kLayDense;
+        const int nstore = min(min(ns, SMAX), dense ? o.S : out.s_max);
+        for (int g = 0; g < nstore; g += NS) {
+            float pend[NS][3];
+            NbBuf<MODE, KMAX> buf[NS];
+            const int cnt = min(NS, nstore - g);
+            float tabs = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) {
+                float tq = 0.0f;
+#pragma unroll
+                for (int x = 0; x < SMAX; ++x)
+                    if (x == g + q) tq = ts_arr[x];
+                pend[q][0] = ray.o[0] + tq * ray.d[0];
+                pend[q][1] = ray.o[1] + tq * ray.d[1];
+                pend[q][2] = ray.o[2] + tq * ray.d[2];
+                if (q < cnt) tabs = fmaxf(tabs, fabsf(tq));
+                buf[q].init();
+            }
+            const float dl1 = fabsf(ray.d[0]) + fabsf(ray.d[1]) + fabsf(ray.d[2]);
+            LaneMask nmask = hmask;
+            if (o.cull) {
+                bool keep_lane = false;
+                if (((emask & ~hmask) >> lane) & 1ull)
+                    keep_lane = !cone_clears(ray, src.cone4(ubl, vbl), src.cone2(ubl, vbl),
+                                             ray.omargin + 6e-7f * tabs * dl1);
+                nmask |= __ballot(keep_lane);
+            }
+            if (MODE != EPCQ_MODE_COMPAT) nmask &= inside_lanes(o, u, v, W, H);
+            LaneMask m = nmask;
+            while (m) {
+                const int l = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1ull;
+                const int ex = lane_bcast(e.x, l), ey = lane_bcast(e.y, l);
+                for (int c0 = 0; c0 < ey; c0 += kWave) {
+                    const int j = c0 + lane;
+                    float dq[NS];
+                    bool aq[NS];
+                    int id = 0;
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        dq[q] = 0.0f;
+                        aq[q] = false;
+                    }
+                    if (j < ey) {
+                        const float4 p = src.point(ex + j);
+                        id = __float_as_int(p.w);
+                        const float r = o.radius ? src.rad(ex + j, p) : o.r0;
+#pragma unroll
+                        for (int q = 0; q < NS; ++q) {
+                            if (q < cnt) {
+                                const float d2 = sqnorm3(pend[q][0] - p.x, pend[q][1] - p.y, pend[q][2] - p.z);
+                                aq[q] = within(o, d2, r);
+                                if (aq[q]) dq[q] = sqrtf(d2);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < NS; ++q) {
+                        uint64_t b = __ballot(aq[q]);
+                        while (b) {
+                            const int x = __ffsll((unsigned long long)b) - 1;
+                            b &= b - 1ull;
+                            buf[q].add(lane_bcast(dq[q], x), lane_bcast(id, x));
+                        }
+                    }
+                }
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    if (q < cnt) write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
+            }
         }
     }
+    if (lane == 0) write_tail<MODE, LAY>(o, out, ri, ns);
 }
 
 // Smallest float s with sqrtf(s) >= r, so that sqrtf(e2) < r <=> e2 < s.
@@ -818,9 +1089,12 @@ struct TileSmem {
     int* rowbase;
     float* rad;  // [cap] radii of the staged points (per-point radius)
     int* total;
+    int* heavy;  // [kMaxTileRays] deferred rays (thread index in the tile), then their count
 };
 
-__device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan& tp, int* total) {
+constexpr int kMaxTileRays = 256;  // threads (rays) of a tile block
+
+__device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan& tp, int* total, int* heavy) {
     TileSmem m;
     const int rr = tp.rw_max * tp.rh_max;
     m.pts = dyn;
@@ -831,6 +1105,7 @@ __device__ __forceinline__ TileSmem carve_tile_smem(float4* dyn, const TilePlan&
     m.rowbase = m.rowlo + tp.rh_max;
     m.rad = reinterpret_cast<float*>(m.rowbase + tp.rh_max + 1);
     m.total = total;
+    m.heavy = heavy;
     return m;
 }
 
@@ -930,9 +1205,36 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
     return total;
 }
 
+// Appends the calling thread to the block's queue of deferred (heavy) rays:
+// heavy[0..n) entries, n in heavy[kMaxTileRays].
+__device__ __forceinline__ void defer_ray(int* heavy, int who) {
+    const int at = atomicAdd(&heavy[kMaxTileRays], 1);
+    heavy[at] = who;
+}
+
+// The block's deferred rays of a tile with top-left ray (u0, v0), each run by
+// one whole wave (run_ray_wave); waves take every nw-th entry.  The queue is
+// read after a barrier; the caller clears it before the next tile.
+template <int MODE, int KMAX, int SMAX, int LAY, class Src>
+__device__ __forceinline__ void run_deferred(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
+                                             const Src& src, int tw, int u0, int v0, int64_t ray_off,
+                                             const int* heavy) {
+    const int n = heavy[kMaxTileRays];
+    const int nw = (int)blockDim.x / kWave;
+    for (int h = (int)threadIdx.x / kWave; h < n; h += nw) {
+        const int t = heavy[h];
+        const int u = u0 + t % tw, v = v0 + t / tw;
+        Ray ray;
+        view_ray(view, u, v, ray);
+        run_ray_wave<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+                                            ray_off + (int64_t)v * view.W + u);
+    }
+}
+
 // One tile of one view: stage its halo region and sample its rays.  Output
 // row of ray (u, v) is ray_off + v * W + u.  Block-wide; no early returns, so
-// persistent kernels can call it in a loop.
+// persistent kernels can call it in a loop.  Rays with many candidates are
+// deferred and then run one per wave.
 // Returns false (block-uniform) when the tile's region did not fit the LDS;
 // the caller then samples the tile from global memory (sample_tile_global).
 template <int MODE, int KMAX, int SMAX, int LAY>
@@ -944,37 +1246,47 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     const int u0 = (tile % tiles_x) * tp.tw;
     const int v0 = (tile / tiles_x) * tp.th;
     int ra, va, RW;
+    if (threadIdx.x == 0) sm.heavy[kMaxTileRays] = 0;  // stage_tile's barriers publish it
     const int staged =
         stage_tile(view, o.xyz, o.radius, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va, RW);
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
+    const SrcLds src{sm.hash, sm.pts, sm.rad, sm.ca, sm.cb, ra, va, RW};
     if (staged >= 0 && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcLds src{sm.hash, sm.pts, sm.rad, sm.ca, sm.cb, ra, va, RW};
-        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
-                                       true);
+        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+                                            ray_off + (int64_t)v * view.W + u, true, true))
+            defer_ray(sm.heavy, (int)threadIdx.x);
     }
+    __syncthreads();
+    if (staged >= 0) run_deferred<MODE, KMAX, SMAX, LAY>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
     __syncthreads();  // the LDS is reused by the block's next tile
     return staged >= 0;
 }
 
-// Every ray of a tile, reading global memory (no LDS, no barriers).
+// Every ray of a tile, reading global memory; called after sample_tile_body
+// returned false (its deferred-ray queue is then empty).  Block-wide.
 template <int MODE, int KMAX, int SMAX, int LAY>
 __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
-                                                   const TilePlan& tp, int tile, int tiles_x, int64_t ray_off) {
+                                                   const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
+                                                   const TileSmem& sm) {
     SampleOpts o = o0;
     if (view.xyz) o.xyz = view.xyz;
-    const int u = (tile % tiles_x) * tp.tw + (int)threadIdx.x % tp.tw;
-    const int v = (tile / tiles_x) * tp.th + (int)threadIdx.x / tp.tw;
+    const int u0 = (tile % tiles_x) * tp.tw, v0 = (tile / tiles_x) * tp.th;
+    const int u = u0 + (int)threadIdx.x % tp.tw;
+    const int v = v0 + (int)threadIdx.x / tp.tw;
+    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
+                        view.cone_b, view.W};
     if (u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
-                            view.cone_b, view.W};
-        run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ray_off + (int64_t)v * view.W + u,
-                                       true);
+        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+                                            ray_off + (int64_t)v * view.W + u, true, true))
+            defer_ray(sm.heavy, (int)threadIdx.x);
     }
+    __syncthreads();
+    run_deferred<MODE, KMAX, SMAX, LAY>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
 }
 
 // Cone of pixel p's points seen from the view origin o (see cone_gap).  The

@@ -19,7 +19,8 @@ from . import csrc as _C
 SHAPES = [(16, 16), (8, 8), (16, 8), (32, 8), (32, 4), (64, 2)]
 CAPS = [3000, 1024, 2048, 3800]
 FLAGS = [0, _C.TUNE_PREFILTER, _C.TUNE_XCD_ORDER]
-DEFAULT = {"tile_w": 16, "tile_h": 16, "tile_cap": 3000, "flags": 0}
+HEAVY = [_C.HEAVY_DEFAULT, 64, 256, 0]
+DEFAULT = {"tile_w": 16, "tile_h": 16, "tile_cap": 3000, "flags": 0, "heavy": _C.HEAVY_DEFAULT}
 
 _cache = {}
 
@@ -64,7 +65,7 @@ def time_ms(fn, reps=3, warmup=1, stream=None):
     return e0.elapsed_time(e1) / reps
 
 
-def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, log=None):
+def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, heavy=HEAVY, log=None):
     """Best launch shape for the workload run(tuning) (which applies the
     tuning dict and enqueues the work once).  Returns (tuning, {str: ms})."""
     if not _cache:
@@ -75,7 +76,7 @@ def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, l
     times = {}
 
     def measure(t):
-        name = f'{t["tile_w"]}x{t["tile_h"]}/cap{t["tile_cap"]}/f{t["flags"]}'
+        name = f'{t["tile_w"]}x{t["tile_h"]}/cap{t["tile_cap"]}/f{t["flags"]}/h{t["heavy"]}'
         if name not in times:
             times[name] = time_ms(lambda: run(t), reps=reps, stream=stream)
             if log:
@@ -95,6 +96,10 @@ def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, l
         t = dict(best, flags=fl)
         if measure(t) < measure(best):
             best = t
+    for hv in heavy:
+        t = dict(best, heavy=hv)
+        if measure(t) < measure(best):
+            best = t
     _cache[full_key] = {"best": best, "times": times}
     _store()
     return dict(best), times
@@ -102,4 +107,5 @@ def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, l
 
 def apply(tuning):
     """Makes `tuning` the process-wide launch shape of the per-view samplers."""
-    _C.set_tuning(tuning["tile_w"], tuning["tile_h"], tuning["tile_cap"], tuning.get("flags", 0))
+    _C.set_tuning(tuning["tile_w"], tuning["tile_h"], tuning["tile_cap"], tuning.get("flags", 0),
+                  tuning.get("heavy", _C.HEAVY_DEFAULT))

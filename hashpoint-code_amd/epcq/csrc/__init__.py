@@ -58,7 +58,7 @@ class _Options(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("tile_cap", ctypes.c_int32),
-                ("flags", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("heavy", ctypes.c_int32)]
 
 
 TUNE_XCD_ORDER = 1
@@ -447,16 +447,22 @@ def quick_query_for_nearby_point(neural_points, cam, options, sorted_point_idx_l
     return out
 
 
-def set_tuning(tile_w=16, tile_h=16, tile_cap=3000, flags=0):
+HEAVY_DEFAULT = 128
+
+
+def set_tuning(tile_w=16, tile_h=16, tile_cap=3000, flags=0, heavy=HEAVY_DEFAULT):
     """Launch shape of the full-image samplers: tile_w x tile_h rays per
-    block, tile_cap points staged in LDS per tile, TUNE_* flags."""
-    _raise_on(_lib.epcq_set_tuning(ctypes.byref(_Tuning(int(tile_w), int(tile_h), int(tile_cap), int(flags)))))
+    block, tile_cap points staged in LDS per tile, TUNE_* flags; rays whose
+    unculled window buckets hold more than `heavy` points are sampled by a
+    whole wave (0: never)."""
+    _raise_on(_lib.epcq_set_tuning(ctypes.byref(_Tuning(int(tile_w), int(tile_h), int(tile_cap), int(flags),
+                                                        int(heavy)))))
 
 
 def get_tuning():
     t = _Tuning()
     _raise_on(_lib.epcq_get_tuning(ctypes.byref(t)))
-    return {"tile_w": t.tile_w, "tile_h": t.tile_h, "tile_cap": t.tile_cap, "flags": t.flags}
+    return {"tile_w": t.tile_w, "tile_h": t.tile_h, "tile_cap": t.tile_cap, "flags": t.flags, "heavy": t.heavy}
 
 
 def squared_threshold(r):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EPCQ_ABI_VERSION 1
+#define EPCQ_ABI_VERSION 2
 
 enum epcq_status {
     EPCQ_OK = 0,
@@ -74,14 +74,19 @@ const char* epcq_last_error(void);
  * tile_cap = points of the tile's halo region staged in LDS (clamped to the
  * 64 KB block limit).  flags: EPCQ_TUNE_XCD_ORDER gives each XCD a
  * contiguous run of tiles; EPCQ_TUNE_PREFILTER enables the cross-product
- * rejection before the exact point-to-ray test.  Process-wide, guarded by a
- * mutex; the autotuner sets it.  Defaults: 16, 16, 3000, 0.
+ * rejection before the exact point-to-ray test.  heavy: a ray whose
+ * unculled window buckets hold more than this many points is sampled by a
+ * whole wave (64 lanes share its candidate walks) after the tile's other
+ * rays, instead of by one lane; 0 never does.  Results are identical either
+ * way.  Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
+ * 16, 16, 3000, 0, 128.
  */
 typedef struct epcq_tuning {
     int32_t tile_w;
     int32_t tile_h;
     int32_t tile_cap;
     int32_t flags;
+    int32_t heavy;
 } epcq_tuning;
 #define EPCQ_TUNE_XCD_ORDER 1
 #define EPCQ_TUNE_PREFILTER 2

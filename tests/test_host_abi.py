@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     assert sorted(C.ABI_SYMBOLS) == names
     for n in names:
         assert getattr(C._lib, n) is not None
-    assert C._lib.epcq_abi_version() == 1
+    assert C._lib.epcq_abi_version() == 2
 
 
 def test_squared_threshold_is_exact():
@@ -80,8 +80,11 @@ def test_tuning_roundtrip_and_validation():
     import epcq.csrc as C
     old = C.get_tuning()
     try:
-        C.set_tuning(32, 4, 1024, C.TUNE_PREFILTER)
-        assert C.get_tuning() == {"tile_w": 32, "tile_h": 4, "tile_cap": 1024, "flags": C.TUNE_PREFILTER}
+        C.set_tuning(32, 4, 1024, C.TUNE_PREFILTER, heavy=7)
+        assert C.get_tuning() == {"tile_w": 32, "tile_h": 4, "tile_cap": 1024, "flags": C.TUNE_PREFILTER,
+                                  "heavy": 7}
+        with pytest.raises(RuntimeError, match="heavy"):
+            C.set_tuning(16, 16, 1024, heavy=-1)
         with pytest.raises(RuntimeError, match="tile_w"):
             C.set_tuning(12, 8, 1024)
         with pytest.raises(RuntimeError, match="tile_w"):
