@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     ap.add_argument("--plain-passes", type=int, default=3, help="batched: timed passes of the per-view forms")
+    ap.add_argument("--heavy", type=int, default=None,
+                    help="per-view workloads: candidate count above which a ray is sampled by a whole wave "
+                         "(default: the fastest of a short sweep on the first view)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     return ap.parse_args()
 
@@ -257,7 +260,7 @@ def bench_batched(args, wl):
                              "batched_speedup": ratio(compact)},
         "stages_ms": stage,
         "samples_per_step": n_samples,
-        "roofline": {"bound": "hbm", "kernel": "batch_sample_kernel + batch_fallback_kernel",
+        "roofline": {"bound": "hbm", "kernel": "batch_sample_kernel",
                      "achieved": sample_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sample_gbs / HBM_PEAK_GBS,
                      "traffic": None, "avg_ms": stage["sample"], "bytes_per_launch": sample_bytes,
                      "bytes_model": "8 B/ray hash + 16 B per in-view point + written outputs per ray"},
@@ -363,6 +366,7 @@ def bench_per_view(args, wl):
     import torch
 
     import epcq.csrc as C
+    from epcq import autotune
     from epcq.epcq import NeuralPoints, RenderOptions
 
     dev = torch.device("cuda:0")
@@ -386,6 +390,22 @@ def bench_per_view(args, wl):
                      dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     ev = []
+
+    # wave-wide sampling threshold: given, or the fastest of a short sweep
+    heavy = args.heavy
+    if heavy is None:
+        cs = views[0]
+        idx, depth, table = npts.pts2plane(cs, opt, workspace=ws)
+
+        def one(h):
+            C.set_tuning(**dict(C.get_tuning(), heavy=h))
+            npts.knn_sampling(cs, opt, idx, table, radius=radius, mode=args.mode, feat=feat, s_max=s_max)
+
+        sweep = {h: autotune.time_ms(lambda: one(h), reps=2) for h in (C.HEAVY_DEFAULT, 512, 2048, 0)}
+        heavy = min(sweep, key=sweep.get)
+        log(f"heavy sweep (ms per view): {sweep} -> {heavy}")
+        del idx, depth, table
+    C.set_tuning(**dict(C.get_tuning(), heavy=heavy))
 
     def step(record):
         for cs in views:
@@ -441,9 +461,9 @@ def bench_per_view(args, wl):
         "data": "synthetic",
         "config": {"workload": wl["name"], "points": n, "views": V, "rays_per_step": rays_per_step,
                    "radius": r_max, "per_point_radius": radius is not None, "K": K, "features": F,
-                   "mode": args.mode, "s_max": s_max, "form": "per-view"},
+                   "mode": args.mode, "s_max": s_max, "form": "per-view", "tuning": C.get_tuning()},
         "latency_per_view": percentiles(build_ms + sample_ms),
-        "roofline": {"bound": "hbm", "kernel": "sample_tile_kernel + sample_tile_fallback_kernel (knn_sampling)",
+        "roofline": {"bound": "hbm", "kernel": "sample_tile_kernel (knn_sampling)",
                      "achieved": sample_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sample_gbs / HBM_PEAK_GBS, "traffic": None, "avg_ms": float(sample_ms.mean()),
                      "bytes_per_launch": sample_bytes},

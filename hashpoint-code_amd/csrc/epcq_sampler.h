@@ -42,8 +42,11 @@
 namespace epcq {
 
 constexpr int kSlotTable = 8;
-// Rays whose hit lanes hold more candidates than this are run by a whole wave.
+// Rays whose hit lanes hold more candidates than this are run by a whole wave
+// when at most kHeavyMinority rays of their wave are (or when they hold 4x as
+// many).
 constexpr int kHeavyCandidates = 128;
+constexpr int kHeavyMinority = 16;
 
 struct SampleOpts {
     int k, ko, L;         // window size, ceil(k/2), k*k
@@ -602,8 +605,8 @@ __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut&
 // left untouched and false returned: the caller then runs it with
 // run_ray_wave, which spreads its candidates over a whole wave.
 template <int MODE, int KMAX, int SMAX, int LAY, class Src>
-__device__ bool run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
-                        int W, int H, int64_t ri, bool valid_ray, bool may_defer) {
+__device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray,
+                                        int u, int v, int W, int H, int64_t ri, bool valid_ray, bool may_defer) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;  // sample points served by one neighbour walk
     constexpr bool dense = LAY == kLayDense;
     int ns = 0;
@@ -614,7 +617,13 @@ __device__ bool run_ray(const SampleOpts& o, const SampleOut& out, const Src& sr
     LaneMask emask = 0;
     int cand = 0;
     const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin, emask, cand) : 0ull;
-    if (may_defer && o.heavy > 0 && cand > o.heavy) return false;
+    if (may_defer && o.heavy > 0) {
+        // only the wave's outliers: a ray above `heavy` while at most
+        // kHeavyMinority of the wave's rays are, or one above 4x `heavy` (a
+        // wave whose rays are all dense is faster one ray per lane)
+        const int above = __popcll(__ballot(cand > o.heavy));
+        if (cand > o.heavy && (above <= kHeavyMinority || cand > 4 * o.heavy)) return false;
+    }
     if (valid_ray && hmask != 0ull) {
         // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
         float tmin = 1e3f, tmax = -1e3f;
@@ -827,8 +836,97 @@ __device__ __forceinline__ float wave_max(float x) {
     return x;
 }
 
+// write_neighbours of the feature layout with the wave's lanes sharing the
+// work (the buffer is the same in every lane): lane b writes neighbour b,
+// lane f the weighted feature f, each value computed exactly as
+// write_neighbours computes it.
+template <int MODE, int KMAX>
+__device__ __forceinline__ void write_neighbours_wave(const SampleOpts& o, const SampleOut& out, const Ray& ray,
+                                                      const NbBuf<MODE, KMAX>& nb, int64_t ri, int so) {
+    constexpr int LAY = kLayFeat;
+    const int lane = __lane_id();
+    const int nkeep = nb.bsize < o.Kc ? nb.bsize : o.Kc;
+    const int64_t row = (ri * out.s_max + so) * (int64_t)o.Kc;
+    // the weights 1 / (d + 1e-6) (epcq.py:436) are recomputed where used (the
+    // same value each time) rather than held in an array
+    float den = 0.0f;
+    if constexpr (LAY == kLayFeat) {
+#pragma unroll
+        for (int b = 0; b < KMAX; ++b)
+            if (b < nkeep) den = den + 1.0f / (nb.bd[b] + 1e-6f);
+        den = den + 1e-6f;
+    }
+    for (int b = lane; b < o.Kc; b += kWave) {
+        int id = -1;
+        float myd = 0.0f;
+#pragma unroll
+        for (int c = 0; c < KMAX; ++c)
+            if (c == b) {
+                id = nb.bi[c];
+                myd = nb.bd[c];
+            }
+        const bool valid = b < KMAX && b < nkeep;
+        out.nb_idx[row + b] = valid ? id : -1;
+        if constexpr (MODE == EPCQ_MODE_KNN) {
+            if (out.nb_t || out.nb_dist) {
+                float t = 0.0f, dist = 0.0f;
+                if (valid) {
+                    const float* p = o.xyz + 3 * (int64_t)id;
+                    float e2;
+                    ray_t_e2(ray, make_float4(p[0], p[1], p[2], 0.0f), t, e2);
+                    dist = sqrtf(e2);
+                }
+                if (out.nb_t) out.nb_t[row + b] = t;
+                if (out.nb_dist) out.nb_dist[row + b] = dist;
+            }
+        }
+        if constexpr (LAY == kLayFeat) out.w[row + b] = valid ? (1.0f / (myd + 1e-6f)) / den : 0.0f;
+    }
+    if constexpr (LAY == kLayFeat) {
+        float* ao = out.agg + (ri * out.s_max + so) * (int64_t)o.F;
+        for (int f = lane; f < o.F; f += kWave) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int b = 0; b < KMAX; ++b) {
+                if (b < nkeep) {
+                    const float wb = (1.0f / (nb.bd[b] + 1e-6f)) / den;
+                    const int64_t at = (int64_t)nb.bi[b] * o.F + f;
+                    const float fv = o.feat_bf16
+                                         ? __uint_as_float((uint32_t) static_cast<const uint16_t*>(o.feat)[at] << 16)
+                                         : static_cast<const float*>(o.feat)[at];
+                    acc = acc + wb * fv;
+                }
+            }
+            ao[f] = acc;
+        }
+    }
+}
+
+// write_tail of the feature layout with the wave's lanes sharing the slots.
+template <int MODE>
+__device__ __forceinline__ void write_tail_wave(const SampleOpts& o, const SampleOut& out, int64_t ri, int ns) {
+    constexpr int LAY = kLayFeat;
+    const int lane = __lane_id();
+    {
+        if (lane == 0) out.n_samples[ri] = ns;
+        const int per = out.s_max - ns;  // empty slots, o.Kc entries each
+        for (int i = lane; i < per * o.Kc; i += kWave) {
+            const int64_t at = (ri * out.s_max + ns + i / o.Kc) * o.Kc + i % o.Kc;
+            out.nb_idx[at] = -1;
+            if constexpr (MODE == EPCQ_MODE_KNN) {
+                if (out.nb_t) out.nb_t[at] = 0.0f;
+                if (out.nb_dist) out.nb_dist[at] = 0.0f;
+            }
+            if constexpr (LAY == kLayFeat) out.w[at] = 0.0f;
+        }
+        for (int s = ns + lane; s < out.s_max; s += kWave) out.sample_t[ri * out.s_max + s] = 0.0f;
+        if constexpr (LAY == kLayFeat)
+            for (int i = lane; i < per * o.F; i += kWave) out.agg[(ri * out.s_max + ns) * o.F + i] = 0.0f;
+    }
+}
+
 template <int MODE, int KMAX, int SMAX, int LAY, class Src>
-__device__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
+__device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
                              int W, int H, int64_t ri) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;
     const int lane = __lane_id();
@@ -1025,14 +1123,19 @@ __device__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Sr
                     }
                 }
             }
-            if (lane == 0) {
+            // feature rows are wide enough to share out; the other layouts
+            // write from lane 0 (sharing them slows the one-lane form's code)
 #pragma unroll
-                for (int q = 0; q < NS; ++q)
-                    if (q < cnt) write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
+            for (int q = 0; q < NS; ++q) {
+                if (q < cnt) {
+                    if constexpr (LAY == kLayFeat) write_neighbours_wave<MODE, KMAX>(o, out, ray, buf[q], ri, g + q);
+                    else if (lane == 0) write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
+                }
             }
         }
     }
-    if (lane == 0) write_tail<MODE, LAY>(o, out, ri, ns);
+    if constexpr (LAY == kLayFeat) write_tail_wave<MODE>(o, out, ri, ns);
+    else if (lane == 0) write_tail<MODE, LAY>(o, out, ri, ns);
 }
 
 // Smallest float s with sqrtf(s) >= r, so that sqrtf(e2) < r <=> e2 < s.

@@ -75,10 +75,10 @@ const char* epcq_last_error(void);
  * 64 KB block limit).  flags: EPCQ_TUNE_XCD_ORDER gives each XCD a
  * contiguous run of tiles; EPCQ_TUNE_PREFILTER enables the cross-product
  * rejection before the exact point-to-ray test.  heavy: a ray whose
- * unculled window buckets hold more than this many points is sampled by a
- * whole wave (64 lanes share its candidate walks) after the tile's other
- * rays, instead of by one lane; 0 never does.  Results are identical either
- * way.  Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
+ * unculled window buckets hold more than this many points (more than 4x this
+ * many when over 16 rays of its wave of 64 exceed it) is sampled by a whole
+ * wave (64 lanes share its candidate walks) after the tile's other rays,
+ * instead of by one lane; 0 never does.  Results are identical either way.  Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
  * 16, 16, 3000, 0, 128.
  */
 typedef struct epcq_tuning {
