@@ -30,6 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hashpoint-code_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 256 CUs x 2.4 GHz x one wave instruction per CU per cycle
+VALU_PEAK_PER_S = 256 * 2.4e9
+# committed rocprofv3 summaries of the batched workload (compat mode): HBM
+# bytes per launch (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and VALU wave
+# instructions per launch of the sampling kernel
+PROFILES = {
+    ("box20-1M-64views-mixed", "compat"): ("profiles/batched_box20-1M-64views_hbm_traffic.txt",
+                                           "profiles/batch_sample_kernel_box20-1M-64views_sq_counters.txt"),
+}
 METRIC = "rays/s search+sample, batched/served vs plain per-view; p50/p99 latency"
 
 
@@ -44,15 +53,36 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="batched: default launch shape, no autotuning")
     ap.add_argument("--no-serve", action="store_true", help="batched: skip the serving sweep")
-    ap.add_argument("--tuning", default=None, help="batched: launch shape tile_w,tile_h,tile_cap,flags[,heavy] (no autotuning)")
+    ap.add_argument("--tuning", default=None, help="launch shape tile_w,tile_h,tile_cap,flags[,heavy] (no autotuning)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     ap.add_argument("--plain-passes", type=int, default=3, help="batched: timed passes of the per-view forms")
     ap.add_argument("--heavy", type=int, default=None,
                     help="per-view workloads: candidate count above which a ray is sampled by a whole wave "
-                         "(default: the fastest of a short sweep on the first view)")
+                         "(skips the autotuning of the launch shape on the first view)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     return ap.parse_args()
+
+
+def profiled(workload, mode):
+    """(HBM bytes per launch, VALU wave instructions per launch, sources) of
+    the sampling kernel from the committed profiles, or Nones."""
+    names = PROFILES.get((workload, mode))
+    if not names:
+        return None, None, None
+    traffic = valu = None
+    try:
+        with open(os.path.join(ROOT, names[0])) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    traffic = float(line.rsplit(":", 1)[1])
+        with open(os.path.join(ROOT, names[1])) as f:
+            for line in f:
+                if line.startswith("SQ_INSTS_VALU"):
+                    valu = float(line.split()[1])
+    except OSError:
+        return None, None, None
+    return traffic, valu, list(names)
 
 
 def log(msg):
@@ -233,6 +263,7 @@ def bench_batched(args, wl):
     compact = rays_per_step / compact_s if compact_s else None
     ratio = lambda x: value / x if x else None
     ms = lambda x: x * 1e3 if x else None
+    traffic, valu, prof_src = profiled(wl["name"], args.mode)
 
     record = {
         "metric": METRIC,
@@ -262,8 +293,13 @@ def bench_batched(args, wl):
         "samples_per_step": n_samples,
         "roofline": {"bound": "hbm", "kernel": "batch_sample_kernel",
                      "achieved": sample_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sample_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "avg_ms": stage["sample"], "bytes_per_launch": sample_bytes,
-                     "bytes_model": "8 B/ray hash + 16 B per in-view point + written outputs per ray"},
+                     "traffic": traffic, "avg_ms": stage["sample"], "bytes_per_launch": sample_bytes,
+                     "bytes_model": "8 B/ray hash + 16 B per in-view point + written outputs per ray",
+                     "traffic_source": prof_src[0] if prof_src else None},
+        "issue_roofline": ({"bound": "valu", "valu_wave_instructions": valu,
+                            "achieved_per_s": valu / (stage["sample"] * 1e-3), "peak_per_s": VALU_PEAK_PER_S,
+                            "frac": valu / (stage["sample"] * 1e-3) / VALU_PEAK_PER_S, "source": prof_src[1]}
+                           if valu else None),
         "build": {"avg_ms": stage["build"], "achieved_GBs": build_gbs, "frac": build_gbs / HBM_PEAK_GBS,
                   "bytes_per_step": build_bytes, "bytes_model": "16 B/point/view + 8 B/pixel"},
     }
@@ -391,21 +427,28 @@ def bench_per_view(args, wl):
     stream = torch.cuda.current_stream(dev)
     ev = []
 
-    # wave-wide sampling threshold: given, or the fastest of a short sweep
-    heavy = args.heavy
-    if heavy is None:
+    # launch shape of the per-view sampler: given, or autotuned on the first view
+    if args.tuning or args.heavy is not None:
+        best = dict(autotune.DEFAULT)
+        if args.tuning:
+            best.update(zip(("tile_w", "tile_h", "tile_cap", "flags", "heavy"),
+                            (int(x) for x in args.tuning.split(","))))
+        if args.heavy is not None:
+            best["heavy"] = args.heavy
+    else:
         cs = views[0]
         idx, depth, table = npts.pts2plane(cs, opt, workspace=ws)
 
-        def one(h):
-            C.set_tuning(**dict(C.get_tuning(), heavy=h))
+        def one(t):
+            autotune.apply(t)
             npts.knn_sampling(cs, opt, idx, table, radius=radius, mode=args.mode, feat=feat, s_max=s_max)
 
-        sweep = {h: autotune.time_ms(lambda: one(h), reps=2) for h in (C.HEAVY_DEFAULT, 512, 2048, 0)}
-        heavy = min(sweep, key=sweep.get)
-        log(f"heavy sweep (ms per view): {sweep} -> {heavy}")
+        t0 = time.perf_counter()
+        best, _ = autotune.tune(f"view/{wl['name']}/{args.mode}", one, reps=2, heavy=(C.HEAVY_DEFAULT, 512, 2048, 0),
+                                log=log)
+        log(f"autotune: best {best} in {time.perf_counter() - t0:.1f} s")
         del idx, depth, table
-    C.set_tuning(**dict(C.get_tuning(), heavy=heavy))
+    autotune.apply(best)
 
     def step(record):
         for cs in views:

@@ -63,7 +63,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxLiveSamples = 16;  // samples per ray the sampler can emit
 
 std::mutex g_tuning_mu;
-epcq_tuning g_tuning{16, 16, 3000, 0, kHeavyCandidates};
+epcq_tuning g_tuning{32, 8, 1024, 0, kHeavyCandidates};
 
 epcq_tuning current_tuning() {
     std::lock_guard<std::mutex> lk(g_tuning_mu);

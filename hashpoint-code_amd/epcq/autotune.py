@@ -17,10 +17,10 @@ import torch
 from . import csrc as _C
 
 SHAPES = [(16, 16), (8, 8), (16, 8), (32, 8), (32, 4), (64, 2)]
-CAPS = [3000, 1024, 2048, 3800]
+CAPS = [1024, 512, 2048, 3000]
 FLAGS = [0, _C.TUNE_PREFILTER, _C.TUNE_XCD_ORDER]
 HEAVY = [_C.HEAVY_DEFAULT, 64, 256, 0]
-DEFAULT = {"tile_w": 16, "tile_h": 16, "tile_cap": 3000, "flags": 0, "heavy": _C.HEAVY_DEFAULT}
+DEFAULT = {"tile_w": 32, "tile_h": 8, "tile_cap": 1024, "flags": 0, "heavy": _C.HEAVY_DEFAULT}
 
 _cache = {}
 

@@ -450,7 +450,7 @@ def quick_query_for_nearby_point(neural_points, cam, options, sorted_point_idx_l
 HEAVY_DEFAULT = 128
 
 
-def set_tuning(tile_w=16, tile_h=16, tile_cap=3000, flags=0, heavy=HEAVY_DEFAULT):
+def set_tuning(tile_w=32, tile_h=8, tile_cap=1024, flags=0, heavy=HEAVY_DEFAULT):
     """Launch shape of the full-image samplers: tile_w x tile_h rays per
     block, tile_cap points staged in LDS per tile, TUNE_* flags; rays whose
     unculled window buckets hold more than `heavy` points are sampled by a

@@ -79,7 +79,7 @@ const char* epcq_last_error(void);
  * many when over 16 rays of its wave of 64 exceed it) is sampled by a whole
  * wave (64 lanes share its candidate walks) after the tile's other rays,
  * instead of by one lane; 0 never does.  Results are identical either way.  Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
- * 16, 16, 3000, 0, 128.
+ * 32, 8, 1024, 0, 128.
  */
 typedef struct epcq_tuning {
     int32_t tile_w;
