@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
     }
 }
 
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, bool PR>
 __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restrict__ h,
                                                            const BatchView* __restrict__ views,
                                                            const int* __restrict__ prefix, SampleOpts o,
@@ -188,11 +188,12 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
         // a tile whose halo region exceeds the LDS reads global memory right
         // away: the densest views come first, so these long tiles overlap
         // with the rest of the batch
-        const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v],
-                                                                            bv.tiles_x, bv.ray_off, sm);
+        const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp,
+                                                                                item - prefix[v], bv.tiles_x,
+                                                                                bv.ray_off, sm);
         if (!staged)
-            sample_tile_global<MODE, KMAX, SMAX, kLayCompact>(o, out, bv.view, tp, item - prefix[v], bv.tiles_x,
-                                                              bv.ray_off, sm);
+            sample_tile_global<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp, item - prefix[v],
+                                                                  bv.tiles_x, bv.ray_off, sm);
     }
 }
 
@@ -294,8 +295,12 @@ template <int MODE, int KMAX, int SMAX>
 void enqueue_sample(epcq_batch* b, const SampleOpts& o, hipStream_t st) {
     const BatchView* dv = b->d_views;
     const int* pt = b->d_prefix + kTile * (b->max_views + 1);  // tiles, densest views first
-    batch_sample_kernel<MODE, KMAX, SMAX><<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(
-        b->d_header, dv, pt, o, b->tp);
+    if (o.radius)
+        batch_sample_kernel<MODE, KMAX, SMAX, true>
+            <<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(b->d_header, dv, pt, o, b->tp);
+    else
+        batch_sample_kernel<MODE, KMAX, SMAX, false>
+            <<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(b->d_header, dv, pt, o, b->tp);
 }
 
 template <int MODE, int KMAX>

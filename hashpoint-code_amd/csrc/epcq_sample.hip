@@ -70,7 +70,7 @@ epcq_tuning current_tuning() {
     return g_tuning;
 }
 
-template <int MODE, int KMAX, int SMAX, int LAY>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
 __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
                                                              int tiles_x) {
     extern __shared__ float4 s_dyn[];
@@ -79,21 +79,21 @@ __global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, Sampl
     const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
     const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     // a tile whose halo region exceeds the LDS reads global memory
-    if (!sample_tile_body<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm))
-        sample_tile_global<MODE, KMAX, SMAX, LAY>(o, out, view, tp, tile, tiles_x, 0, sm);
+    if (!sample_tile_body<MODE, KMAX, SMAX, LAY, PR>(o, out, view, tp, tile, tiles_x, 0, sm))
+        sample_tile_global<MODE, KMAX, SMAX, LAY, PR>(o, out, view, tp, tile, tiles_x, 0, sm);
 }
 
 // One thread per listed ray; rays with many candidates are deferred and then
 // run one per wave (the block's waves share the queue).
-template <int MODE, int KMAX, int SMAX, int LAY>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
 __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, SampleOut out, ViewArgs view,
                                                              const int* __restrict__ rays, int64_t R) {
     __shared__ int s_heavy[kMaxTileRays + 1];
     static_assert(kBlock <= kMaxTileRays, "queue holds one entry per thread");
     if (threadIdx.x == 0) s_heavy[kMaxTileRays] = 0;
     __syncthreads();
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a, view.cone_b,
-                        view.W};
+    const SrcGlobal<PR> src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
+                            view.cone_b, view.W};
     const int64_t r0 = (int64_t)blockIdx.x * kBlock;
     const int64_t ri = r0 + threadIdx.x;
     if (ri < R) {
@@ -278,19 +278,33 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     view.prad = prad;
 }
 
-template <int MODE, int KMAX, int SMAX, int LAY>
-void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
+void launch_kernels(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                 const TilePlan& tp, hipStream_t st) {
     if (rays) {
-        sample_list_kernel<MODE, KMAX, SMAX, LAY><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
+        sample_list_kernel<MODE, KMAX, SMAX, LAY, PR><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
             o, out, view, rays, R);
     } else {
         const int tiles_x = (int)ceil_div(view.W, tp.tw);
         const int tiles = tiles_x * (int)ceil_div(view.H, tp.th);
         const dim3 block(tp.tw * tp.th);
-        sample_tile_kernel<MODE, KMAX, SMAX, LAY><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp,
-                                                                                          tiles_x);
+        sample_tile_kernel<MODE, KMAX, SMAX, LAY, PR><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp,
+                                                                                              tiles_x);
     }
+}
+
+// Per-point radius is a kernel property (PR); the dense drop-in layout has
+// none (quick_sampling takes a uniform radius).
+template <int MODE, int KMAX, int SMAX, int LAY>
+void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                const TilePlan& tp, hipStream_t st) {
+    if constexpr (LAY != kLayDense) {
+        if (o.radius) {
+            launch_kernels<MODE, KMAX, SMAX, LAY, true>(o, out, view, rays, R, tp, st);
+            return;
+        }
+    }
+    launch_kernels<MODE, KMAX, SMAX, LAY, false>(o, out, view, rays, R, tp, st);
 }
 
 template <int MODE, int KMAX, int SMAX>

@@ -89,8 +89,11 @@ constexpr int kLayFeat = 2;
 
 // Candidate sources.  Per-pixel bucket cone (see bucket_cone_body): ca =
 // (axis, P), cb = (Q, B); point(j) = (x, y, z, index bits) of list position
-// j; rad(j, p) = its radius when radii are per point (SampleOpts::radius).
+// j; rad(j, p) = its radius.  PR: radii are per point (SampleOpts::radius);
+// a kernel without them carries neither the radius pointers nor the test.
+template <bool PR>
 struct SrcGlobal {
+    static constexpr bool kPR = PR;
     const int2* hash;
     const int* sorted_idx;
     const float* xyz;
@@ -114,7 +117,9 @@ struct SrcGlobal {
     }
 };
 
+template <bool PR>
 struct SrcLds {
+    static constexpr bool kPR = PR;
     const int2* hash;
     const float4* pts;
     const float* prad;  // staged radii (per-point radius only)
@@ -185,8 +190,17 @@ __device__ __forceinline__ bool cone_clears(const Ray& ray, const float4& ca, co
 
 // dist < radius for a squared distance, exact (see the file comment); r is
 // the point's radius when radii are per point.
+template <bool PR>
 __device__ __forceinline__ bool within(const SampleOpts& o, float e2, float r) {
-    return o.radius ? sqrtf(e2) < r : e2 < o.r2lim;
+    if constexpr (PR) return sqrtf(e2) < r;
+    else return e2 < o.r2lim;
+}
+
+// Radius of the candidate at list position j (point p).
+template <class Src>
+__device__ __forceinline__ float point_radius(const SampleOpts& o, const Src& src, int j, const float4& p) {
+    if constexpr (Src::kPR) return src.rad(j, p);
+    else return o.r0;
 }
 
 // Lane sets of one ray: bit l for window lane l (k*k lanes, at most 64).
@@ -297,12 +311,12 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
-            const float r = o.radius ? src.rad(e.x + j, p) : o.r0;
+            const float r = point_radius(o, src, e.x + j, p);
             if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))
                 continue;
             float t, e2;
             ray_t_e2(ray, p, t, e2);
-            if (within(o, e2, r)) {
+            if (within<Src::kPR>(o, e2, r)) {
                 ++kept;
                 fn(t, e2);
             }
@@ -432,12 +446,12 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
                                                NbBuf<MODE, KMAX> (&buf)[NS]) {
     auto visit = [&](const float4& p, int jp) {
         const int id = __float_as_int(p.w);
-        const float r = o.radius ? src.rad(jp, p) : o.r0;
+        const float r = point_radius(o, src, jp, p);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             if (s < ns) {
                 const float d2 = sqnorm3(sp[s][0] - p.x, sp[s][1] - p.y, sp[s][2] - p.z);
-                if (within(o, d2, r)) buf[s].add(sqrtf(d2), id);
+                if (within<Src::kPR>(o, d2, r)) buf[s].add(sqrtf(d2), id);
             }
         }
     };
@@ -960,11 +974,11 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                 float t = 0.0f, e2 = 0.0f;
                 if (j < ey) {
                     const float4 p = src.point(ex + j);
-                    const float r = o.radius ? src.rad(ex + j, p) : o.r0;
+                    const float r = point_radius(o, src, ex + j, p);
                     if (!(o.prefilter &&
                           surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))) {
                         ray_t_e2(ray, p, t, e2);
-                        hit = within(o, e2, r);
+                        hit = within<Src::kPR>(o, e2, r);
                     }
                 }
                 const uint64_t hb = __ballot(hit);
@@ -1102,12 +1116,12 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                     if (j < ey) {
                         const float4 p = src.point(ex + j);
                         id = __float_as_int(p.w);
-                        const float r = o.radius ? src.rad(ex + j, p) : o.r0;
+                        const float r = point_radius(o, src, ex + j, p);
 #pragma unroll
                         for (int q = 0; q < NS; ++q) {
                             if (q < cnt) {
                                 const float d2 = sqnorm3(pend[q][0] - p.x, pend[q][1] - p.y, pend[q][2] - p.z);
-                                aq[q] = within(o, d2, r);
+                                aq[q] = within<Src::kPR>(o, d2, r);
                                 if (aq[q]) dq[q] = sqrtf(d2);
                             }
                         }
@@ -1340,7 +1354,7 @@ __device__ __forceinline__ void run_deferred(const SampleOpts& o, const SampleOu
 // deferred and then run one per wave.
 // Returns false (block-uniform) when the tile's region did not fit the LDS;
 // the caller then samples the tile from global memory (sample_tile_global).
-template <int MODE, int KMAX, int SMAX, int LAY>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
 __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                  const TileSmem& sm) {
@@ -1351,10 +1365,11 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     int ra, va, RW;
     if (threadIdx.x == 0) sm.heavy[kMaxTileRays] = 0;  // stage_tile's barriers publish it
     const int staged =
-        stage_tile(view, o.xyz, o.radius, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va, RW);
+        stage_tile(view, o.xyz, PR ? o.radius : nullptr, o.cull, o.k, o.ko, u0, v0, tp.tw, tp.th, tp.cap, sm, ra, va,
+                   RW);
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
-    const SrcLds src{sm.hash, sm.pts, sm.rad, sm.ca, sm.cb, ra, va, RW};
+    const SrcLds<PR> src{sm.hash, sm.pts, sm.rad, sm.ca, sm.cb, ra, va, RW};
     if (staged >= 0 && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
@@ -1370,7 +1385,7 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
 
 // Every ray of a tile, reading global memory; called after sample_tile_body
 // returned false (its deferred-ray queue is then empty).  Block-wide.
-template <int MODE, int KMAX, int SMAX, int LAY>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
 __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                    const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                    const TileSmem& sm) {
@@ -1379,7 +1394,7 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
     const int u0 = (tile % tiles_x) * tp.tw, v0 = (tile / tiles_x) * tp.th;
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
-    const SrcGlobal src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
+    const SrcGlobal<PR> src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
                         view.cone_b, view.W};
     if (u < view.W && v < view.H) {
         Ray ray;
