@@ -143,14 +143,11 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         const int b = item - prefix[v];
         fine_body(bv.b, b, s_mem, s_wave);
         const int bs = bv.b.bstart[b], tot = bv.b.btotal[b];
-        float4* pts = const_cast<float4*>(bv.view.pts);
-        const float* xyz = bv.b.xyz;
-        float* prad = const_cast<float*>(bv.view.prad);
-        for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock) {
-            const int id = bv.b.out_idx[i];
-            const float* q = xyz + 3 * (int64_t)id;
-            pts[i] = make_float4(q[0], q[1], q[2], __int_as_float(id));
-            if (radius) prad[i] = radius[id];
+        // fine_body wrote the pixel-ordered point copy from the binned records
+        const float4* pts = bv.view.pts;
+        if (radius) {
+            float* prad = const_cast<float*>(bv.view.prad);
+            for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock) prad[i] = radius[bv.b.out_idx[i]];
         }
         __syncthreads();
         const int64_t p0 = (int64_t)b << bv.b.ts;
@@ -251,7 +248,7 @@ struct ViewLayout {
 
 ViewLayout layout_view(const epcq_view_desc& d) {
     ViewLayout L{};
-    L.bp = plan_build(d.n, d.height, d.width);
+    L.bp = plan_build(d.n, d.height, d.width, true);
     L.P = (int64_t)d.width * d.height;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -508,7 +505,7 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         int* out_idx = reinterpret_cast<int*>(vb + L[i].idx);
         int* out_hash = reinterpret_cast<int*>(vb + L[i].hash);
         bv.b = make_build_view(L[i].bp, d.xyz, d.n, d_w2c, d_K, d.height, d.width, 0, vb + L[i].ws, out_idx, nullptr,
-                               out_hash);
+                               out_hash, reinterpret_cast<float4*>(vb + L[i].pts));
         bv.view = ViewArgs{d_c2w, d.fx, d.fy, d.cx, d.cy, d.width, d.height, reinterpret_cast<const int2*>(out_hash),
                            out_idx, reinterpret_cast<const float4*>(vb + L[i].ca),
                            reinterpret_cast<const float2*>(vb + L[i].cb), reinterpret_cast<const float4*>(vb + L[i].pts),

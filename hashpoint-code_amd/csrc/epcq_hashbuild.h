@@ -156,12 +156,14 @@ struct BuildView {
     int* part;    // [nseg][nbins]
     int* bstart;  // [nbins + 1]
     int* btotal;  // [nbins]
-    int* tidx;    // [n] binned point index
+    int* tidx;    // [n] binned point index (null when tpt is used)
+    float4* tpt;  // [n] binned (x, y, z, index bits) records, or null
     float* tdep;  // [n] binned depth
     uint16_t* tfine;  // [n] pixel within bin
     int* out_idx;     // [n]
     float* out_depth; // [n] (may be null: not written)
     int* out_hash;    // [P, 2]
+    float4* out_pts;  // [n] pixel-ordered (x, y, z, index bits) copy (with tpt), or null
 };
 
 // Launch geometry of the build for (n, H, W) and its workspace layout.
@@ -174,9 +176,13 @@ struct BuildPlan {
     int nblk;
     int nseg;
     size_t off_hist, off_part, off_start, off_total, off_tidx, off_tdep, off_tfine, bytes;
+    bool points;
 };
 
-inline BuildPlan plan_build(int64_t n, int H, int W) {
+// points: the bin pass carries each point's coordinates with its index (a
+// 16-byte record) so that the fine pass can write the pixel-ordered point
+// copy without gathering from xyz; no depth is binned then.
+inline BuildPlan plan_build(int64_t n, int H, int W, bool points = false) {
     BuildPlan bp{};
     bp.P = (int64_t)H * W;
     // bins of at most 2^kMaxTileShift pixels, fewer for dense views so that a
@@ -204,8 +210,9 @@ inline BuildPlan plan_build(int64_t n, int H, int W) {
     bp.off_part = take((size_t)bp.nseg * bp.nbins * 4);
     bp.off_start = take((size_t)(bp.nbins + 1) * 4);
     bp.off_total = take((size_t)bp.nbins * 4);
-    bp.off_tidx = take((size_t)n * 4);
-    bp.off_tdep = take((size_t)n * 4);
+    bp.off_tidx = take((size_t)n * (points ? 16 : 4));
+    bp.off_tdep = take(points ? 0 : (size_t)n * 4);
+    bp.points = points;
     bp.off_tfine = take((size_t)n * 2);
     bp.bytes = off;
     return bp;
@@ -214,7 +221,7 @@ inline BuildPlan plan_build(int64_t n, int H, int W) {
 // BuildView of plan bp over workspace ws.
 inline BuildView make_build_view(const BuildPlan& bp, const float* xyz, int64_t n, const float* w2c,
                                  const float* K, int H, int W, int sort, void* ws, int* out_idx,
-                                 float* out_depth, int* out_hash) {
+                                 float* out_depth, int* out_hash, float4* out_pts = nullptr) {
     char* w = static_cast<char*>(ws);
     BuildView v{};
     v.xyz = xyz;
@@ -235,12 +242,14 @@ inline BuildView make_build_view(const BuildPlan& bp, const float* xyz, int64_t 
     v.part = reinterpret_cast<int*>(w + bp.off_part);
     v.bstart = reinterpret_cast<int*>(w + bp.off_start);
     v.btotal = reinterpret_cast<int*>(w + bp.off_total);
-    v.tidx = reinterpret_cast<int*>(w + bp.off_tidx);
-    v.tdep = reinterpret_cast<float*>(w + bp.off_tdep);
+    v.tidx = bp.points ? nullptr : reinterpret_cast<int*>(w + bp.off_tidx);
+    v.tpt = bp.points ? reinterpret_cast<float4*>(w + bp.off_tidx) : nullptr;
+    v.tdep = bp.points ? nullptr : reinterpret_cast<float*>(w + bp.off_tdep);
     v.tfine = reinterpret_cast<uint16_t*>(w + bp.off_tfine);
     v.out_idx = out_idx;
     v.out_depth = out_depth;
     v.out_hash = out_hash;
+    v.out_pts = bp.points ? out_pts : nullptr;
     return v;
 }
 
@@ -335,14 +344,20 @@ __device__ __forceinline__ void bin_body(const BuildView& bv, int blk, int* s_cu
     const float* xyz = bv.xyz;
     for (int64_t r0 = base; r0 < end; r0 += kBuildBlock) {
         const int64_t i = r0 + threadIdx.x;
-        float depth = 0.0f;
+        float depth = 0.0f, x = 0.0f, y = 0.0f, z = 0.0f;
         int key = pp.P;
-        if (i < end) key = project_key(pp, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], depth);
+        if (i < end) {
+            x = xyz[3 * i];
+            y = xyz[3 * i + 1];
+            z = xyz[3 * i + 2];
+            key = project_key(pp, x, y, z, depth);
+        }
         const bool active = key < pp.P;
         const uint32_t bin = active ? ((uint32_t)key >> bv.ts) : 0u;
         const int pos = ordered_append(s_cur, bin, bv.bin_bits, active);
         if (active) {
-            bv.tidx[pos] = (int)i;
+            if (bv.tpt) bv.tpt[pos] = make_float4(x, y, z, __int_as_float((int)i));
+            else bv.tidx[pos] = (int)i;
             if (bv.out_depth) bv.tdep[pos] = depth;  // depths only when they are an output
             bv.tfine[pos] = (uint16_t)((uint32_t)key & fmask);
         }
@@ -383,8 +398,14 @@ __device__ __forceinline__ void fine_body(const BuildView& bv, int b, int* s_mem
         const uint32_t f = active ? bv.tfine[i] : 0u;
         const int pos = ordered_append(s_cur, f, ts, active);
         if (active) {
-            bv.out_idx[bs + pos] = bv.tidx[i];
-            if (out_depth) out_depth[bs + pos] = bv.tdep[i];
+            if (bv.tpt) {
+                const float4 q = bv.tpt[i];
+                bv.out_idx[bs + pos] = __float_as_int(q.w);
+                bv.out_pts[bs + pos] = q;
+            } else {
+                bv.out_idx[bs + pos] = bv.tidx[i];
+                if (out_depth) out_depth[bs + pos] = bv.tdep[i];
+            }
         }
     }
     if (bv.sort && out_depth) {
