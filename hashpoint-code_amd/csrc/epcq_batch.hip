@@ -13,10 +13,11 @@
 // Pipeline (all views at once, one launch per stage):
 //   reset -> count -> col_partial -> bin_total -> bin_scan -> col_apply -> bin -> fine
 //   (the fused hash-table build of epcq_build.hip, per (view, chunk) / (view,
-//   bin) item) -> bounds (per-pixel bucket cones, pixel-ordered points) ->
+//   bin) item; bin carries 16-byte point records, and fine also writes the
+//   bin's pixel-ordered point copy and its pixels' bucket cones) ->
 //   sample (one (view, tile) item at a time from an atomic work counter,
 //   heaviest views first; a tile whose halo exceeds the LDS reads global
-//   memory).
+//   memory; a wave's outlier rays are run wave-wide after the tile).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

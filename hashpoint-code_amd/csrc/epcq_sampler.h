@@ -1449,12 +1449,17 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
             rb = j == 0 ? rj : fmaxf(rb, rj);
         }
         const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
-        const float n = sqrtf(sqnorm3(w0, w1, w2));
-        if (!(n > 0.0f && n < never)) {
+        const float x = sqnorm3(w0, w1, w2);
+        if (!(x > 1e-30f && x < never)) {  // at the origin, tiny or not finite: a cone that never culls
             ok = false;
             continue;
         }
-        const float u0 = w0 / n, u1 = w1 / n, u2 = w2 / n;
+        // unit direction from the hardware reciprocal square root (1 ulp):
+        // the cone is a bound, not a result, and its slack covers the few
+        // ulp this leaves in n and u
+        const float inv = __builtin_amdgcn_rsqf(x);
+        const float n = x * inv;
+        const float u0 = w0 * inv, u1 = w1 * inv, u2 = w2 * inv;
         if (j == 0) {
             f0 = u0;
             f1 = u1;
@@ -1477,16 +1482,17 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
     for (int j = 0; j < e.y; ++j) {
         const float4 q = pts_in ? pts_in[e.x + j] : (pts ? pts[e.x + j] : point(j));
         const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
-        const float n = sqrtf(sqnorm3(w0, w1, w2));
-        const float u0 = w0 / n, u1 = w1 / n, u2 = w2 / n;
+        const float inv = __builtin_amdgcn_rsqf(sqnorm3(w0, w1, w2));
+        const float u0 = w0 * inv, u1 = w1 * inv, u2 = w2 * inv;
         cmin = fminf(cmin, fabsf(dot3(u0, u1, u2, a0, a1, a2)));
         const float x0 = u1 * a2 - u2 * a1, x1 = u2 * a0 - u0 * a2, x2 = u0 * a1 - u1 * a0;
         smax = fmaxf(smax, sqrtf(sqnorm3(x0, x1, x2)));
     }
-    // rho, cos phi rounded down, sin phi up (a few ulp of error each)
-    const float rl = rho * (1.0f - 1e-6f);
-    cone_a[p] = make_float4(a0, a1, a2, rl * (cmin - 1e-6f));
-    cone_b[p] = make_float2(rl * (smax + 1e-6f), rb * (1.0f + 1e-6f) + 4e-6f * rho);
+    // rho, cos phi rounded down, sin phi up: each carries a few ulp of
+    // error (reciprocal square root, products, sums), well inside 2e-6
+    const float rl = rho * (1.0f - 2e-6f);
+    cone_a[p] = make_float4(a0, a1, a2, rl * (cmin - 2e-6f));
+    cone_b[p] = make_float2(rl * (smax + 2e-6f), rb * (1.0f + 1e-6f) + 4e-6f * rho);
 }
 
 }  // namespace epcq
