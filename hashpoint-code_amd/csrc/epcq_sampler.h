@@ -722,6 +722,9 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
             float T = 1.0f;
             int prev = -1;
             while (true) {
+                // the walk's end does not depend on the next slot's sums, so
+                // it is tested before a refill could be needed for them
+                if (T < 1e-3f || ns >= o.S) break;
                 const int s = next_slot(seen2, prev + 1);
                 if (s >= kMaxSlots) break;
                 prev = s;
@@ -759,7 +762,6 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
                         bsum = tsum[e];
                         bcnt = tcnt[e];
                     }
-                if (T < 1e-3f || ns >= o.S) break;
                 const float sdf = bsum / (bcnt + 1e-6f);
                 const float alpha = o.gamma * expf_cw(-sdf / o.alpha_p);
                 const float wgt = alpha * T;
