@@ -133,6 +133,36 @@ struct SrcLds {
     __device__ __forceinline__ float4 point(int j) const { return pts[j]; }
 };
 
+// A tile whose halo points do not fit the LDS: the halo's hash entries (with
+// their list positions in the view, not rebased) and cones are in LDS, the
+// points come from global memory (the pixel-ordered copy, or a gather by
+// index when there is none).
+template <bool PR>
+struct SrcHybrid {
+    static constexpr bool kPR = PR;
+    const int2* hash;  // LDS, halo region
+    const float4* cone_a;
+    const float2* cone_b;
+    int ra, va, RW;
+    const float4* pts;  // global
+    const float* prad;
+    const int* sorted_idx;
+    const float* xyz;
+    const float* radius;
+    __device__ __forceinline__ int2 bucket(int ub, int vb) const { return hash[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float4 cone4(int ub, int vb) const { return cone_a[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float2 cone2(int ub, int vb) const { return cone_b[(vb - va) * RW + (ub - ra)]; }
+    __device__ __forceinline__ float4 point(int j) const {
+        if (pts) return pts[j];
+        const int id = sorted_idx[j];
+        const float* p = xyz + 3 * (int64_t)id;
+        return make_float4(p[0], p[1], p[2], __int_as_float(id));
+    }
+    __device__ __forceinline__ float rad(int j, const float4& p) const {
+        return prad ? prad[j] : radius[__float_as_int(p.w)];
+    }
+};
+
 // PointRayInterset up to the squared distance: t and |p - (o + t d)|^2.
 __device__ __forceinline__ void ray_t_e2(const Ray& r, const float4& p, float& t, float& e2) {
     const float w0 = p.x - r.o[0], w1 = p.y - r.o[1], w2 = p.z - r.o[2];
@@ -1385,8 +1415,10 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     return staged >= 0;
 }
 
-// Every ray of a tile, reading global memory; called after sample_tile_body
-// returned false (its deferred-ray queue is then empty).  Block-wide.
+// Every ray of a tile whose halo points did not fit the LDS; called right
+// after sample_tile_body returned false, so the LDS still holds the halo's
+// hash entries (not rebased) and cones, and its deferred-ray queue is empty.
+// Points are read from global memory.  Block-wide.
 template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
 __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                    const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
@@ -1396,8 +1428,10 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
     const int u0 = (tile % tiles_x) * tp.tw, v0 = (tile / tiles_x) * tp.th;
     const int u = u0 + (int)threadIdx.x % tp.tw;
     const int v = v0 + (int)threadIdx.x / tp.tw;
-    const SrcGlobal<PR> src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
-                        view.cone_b, view.W};
+    int ra, va, RW, RH;
+    tile_region(view.W, view.H, o.k, o.ko, u0, v0, tp.tw, tp.th, ra, va, RW, RH);
+    const SrcHybrid<PR> src{sm.hash, sm.ca, sm.cb, ra, va, RW, view.pts, view.prad, view.sorted_idx, o.xyz,
+                            o.radius};
     if (u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
