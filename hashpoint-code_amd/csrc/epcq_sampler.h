@@ -1460,11 +1460,9 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
                                                  float4* __restrict__ pts, float* __restrict__ prad) {
     const int2 e = hash[p];
     const float never = __builtin_inff();
-    if (e.y <= 0) {
-        cone_a[p] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-        cone_b[p] = make_float2(0.0f, never);
-        return;
-    }
+    // an empty pixel's cone is never read (every reader tests the bucket's
+    // count first), so it is not written
+    if (e.y <= 0) return;
     auto point = [&](int j) {
         if (pts_in) return pts_in[e.x + j];
         const int id = sorted_idx[e.x + j];
