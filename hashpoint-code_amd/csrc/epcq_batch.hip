@@ -148,7 +148,8 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         const float4* pts = bv.view.pts;
         if (radius) {
             float* prad = const_cast<float*>(bv.view.prad);
-            for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock) prad[i] = radius[bv.b.out_idx[i]];
+            for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock)
+                prad[i] = point_threshold(radius[bv.b.out_idx[i]]);
         }
         __syncthreads();
         const int64_t p0 = (int64_t)b << bv.b.ts;

@@ -29,10 +29,11 @@
 // (SrcLds); each kernel instantiates exactly one, so LDS reads stay ds_read
 // instructions.
 //
-// With a uniform radius r the hit test sqrtf(e2) < r is evaluated exactly as
-// e2 < r2lim, where r2lim is the smallest float whose correctly rounded
-// square root is >= r (sqrt is monotone), so square roots are only taken for
-// hits.
+// The hit test sqrtf(e2) < r is evaluated exactly as e2 < lim, where lim is
+// the smallest float whose correctly rounded square root is >= r (sqrt is
+// monotone): r2lim for a uniform radius, and with per-point radii each
+// point's own threshold, stored with the pixel-ordered points.  Square roots
+// are only taken for hits.
 #pragma once
 
 #include <climits>
@@ -87,9 +88,25 @@ constexpr int kLayDense = 0;
 constexpr int kLayCompact = 1;
 constexpr int kLayFeat = 2;
 
+// Smallest float s with sqrtf(s) >= r, so that sqrtf(e2) < r <=> e2 < s
+// (sqrt is monotone); the device twin of squared_threshold.  0 for r <= 0 or
+// NaN (no distance is below r), +inf for r = +inf.
+__device__ __forceinline__ float point_threshold(float r) {
+    if (!(r > 0.0f)) return 0.0f;
+    if (r == __builtin_inff()) return r;
+    float s = r * r;
+    if (sqrtf(s) >= r) {
+        while (s > 0.0f && sqrtf(nextafterf(s, 0.0f)) >= r) s = nextafterf(s, 0.0f);
+    } else {
+        while (!(sqrtf(s) >= r)) s = nextafterf(s, __builtin_inff());
+    }
+    return s;
+}
+
 // Candidate sources.  Per-pixel bucket cone (see bucket_cone_body): ca =
 // (axis, P), cb = (Q, B); point(j) = (x, y, z, index bits) of list position
-// j; rad(j, p) = its radius.  PR: radii are per point (SampleOpts::radius);
+// j; rad(j, p) = the squared threshold of its radius (point_threshold).  PR:
+// radii are per point (SampleOpts::radius);
 // a kernel without them carries neither the radius pointers nor the test.
 template <bool PR>
 struct SrcGlobal {
@@ -98,7 +115,7 @@ struct SrcGlobal {
     const int* sorted_idx;
     const float* xyz;
     const float4* pts;   // pixel-ordered (x, y, z, index bits) or nullptr (gather)
-    const float* prad;   // pixel-ordered radii or nullptr (gather)
+    const float* prad;   // pixel-ordered radius thresholds or nullptr (gather)
     const float* radius;  // per-point radii by index (when prad is null)
     const float4* cone_a;
     const float2* cone_b;
@@ -113,7 +130,7 @@ struct SrcGlobal {
         return make_float4(p[0], p[1], p[2], __int_as_float(id));
     }
     __device__ __forceinline__ float rad(int j, const float4& p) const {
-        return prad ? prad[j] : radius[__float_as_int(p.w)];
+        return prad ? prad[j] : point_threshold(radius[__float_as_int(p.w)]);
     }
 };
 
@@ -122,7 +139,7 @@ struct SrcLds {
     static constexpr bool kPR = PR;
     const int2* hash;
     const float4* pts;
-    const float* prad;  // staged radii (per-point radius only)
+    const float* prad;  // staged radius thresholds (per-point radius only)
     const float4* cone_a;
     const float2* cone_b;
     int ra, va, RW;
@@ -159,7 +176,7 @@ struct SrcHybrid {
         return make_float4(p[0], p[1], p[2], __int_as_float(id));
     }
     __device__ __forceinline__ float rad(int j, const float4& p) const {
-        return prad ? prad[j] : radius[__float_as_int(p.w)];
+        return prad ? prad[j] : point_threshold(radius[__float_as_int(p.w)]);
     }
 };
 
@@ -218,18 +235,25 @@ __device__ __forceinline__ bool cone_clears(const Ray& ray, const float4& ca, co
     return sq * (ca.w * ca.w) > x * x;
 }
 
-// dist < radius for a squared distance, exact (see the file comment); r is
-// the point's radius when radii are per point.
-template <bool PR>
-__device__ __forceinline__ bool within(const SampleOpts& o, float e2, float r) {
-    if constexpr (PR) return sqrtf(e2) < r;
-    else return e2 < o.r2lim;
+// dist < radius for a squared distance, exact (see the file comment): lim is
+// the squared threshold of the radius (o.r2lim, or the point's own with
+// per-point radii), so no square root is taken.
+__device__ __forceinline__ bool within(float e2, float lim) { return e2 < lim; }
+
+// Squared threshold of the radius of the candidate at list position j (point
+// p): with per-point radii the sources hold each point's threshold.
+template <class Src>
+__device__ __forceinline__ float point_limit(const SampleOpts& o, const Src& src, int j, const float4& p) {
+    if constexpr (Src::kPR) return src.rad(j, p);
+    else return o.r2lim;
 }
 
-// Radius of the candidate at list position j (point p).
-template <class Src>
-__device__ __forceinline__ float point_radius(const SampleOpts& o, const Src& src, int j, const float4& p) {
-    if constexpr (Src::kPR) return src.rad(j, p);
+// A radius at least the one whose squared threshold is lim, for the
+// conservative prefilter (the hardware square root's ulp is inside its
+// margin).
+template <bool PR>
+__device__ __forceinline__ float radius_bound(const SampleOpts& o, float lim) {
+    if constexpr (PR) return __builtin_amdgcn_sqrtf(lim) * 1.000001f;
     else return o.r0;
 }
 
@@ -341,12 +365,13 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
-            const float r = point_radius(o, src, e.x + j, p);
-            if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))
+            const float lim = point_limit(o, src, e.x + j, p);
+            if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2],
+                                          radius_bound<Src::kPR>(o, lim), ray.omargin))
                 continue;
             float t, e2;
             ray_t_e2(ray, p, t, e2);
-            if (within<Src::kPR>(o, e2, r)) {
+            if (within(e2, lim)) {
                 ++kept;
                 fn(t, e2);
             }
@@ -476,12 +501,12 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
                                                NbBuf<MODE, KMAX> (&buf)[NS]) {
     auto visit = [&](const float4& p, int jp) {
         const int id = __float_as_int(p.w);
-        const float r = point_radius(o, src, jp, p);
+        const float lim = point_limit(o, src, jp, p);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             if (s < ns) {
                 const float d2 = sqnorm3(sp[s][0] - p.x, sp[s][1] - p.y, sp[s][2] - p.z);
-                if (within<Src::kPR>(o, d2, r)) buf[s].add(sqrtf(d2), id);
+                if (within(d2, lim)) buf[s].add(sqrtf(d2), id);
             }
         }
     };
@@ -1006,11 +1031,12 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                 float t = 0.0f, e2 = 0.0f;
                 if (j < ey) {
                     const float4 p = src.point(ex + j);
-                    const float r = point_radius(o, src, ex + j, p);
+                    const float lim = point_limit(o, src, ex + j, p);
                     if (!(o.prefilter &&
-                          surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2], r, ray.omargin))) {
+                          surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2],
+                                     radius_bound<Src::kPR>(o, lim), ray.omargin))) {
                         ray_t_e2(ray, p, t, e2);
-                        hit = within<Src::kPR>(o, e2, r);
+                        hit = within(e2, lim);
                     }
                 }
                 const uint64_t hb = __ballot(hit);
@@ -1148,12 +1174,12 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                     if (j < ey) {
                         const float4 p = src.point(ex + j);
                         id = __float_as_int(p.w);
-                        const float r = point_radius(o, src, ex + j, p);
+                        const float lim = point_limit(o, src, ex + j, p);
 #pragma unroll
                         for (int q = 0; q < NS; ++q) {
                             if (q < cnt) {
                                 const float d2 = sqnorm3(pend[q][0] - p.x, pend[q][1] - p.y, pend[q][2] - p.z);
-                                aq[q] = within<Src::kPR>(o, d2, r);
+                                aq[q] = within(d2, lim);
                                 if (aq[q]) dq[q] = sqrtf(d2);
                             }
                         }
@@ -1216,7 +1242,7 @@ struct ViewArgs {
     const float4* cone_a;  // bucket cones (bucket_cone_body) or nullptr
     const float2* cone_b;
     const float4* pts;     // pixel-ordered points (bucket_cone_body) or nullptr
-    const float* prad;     // pixel-ordered radii (per-point radius) or nullptr
+    const float* prad;     // pixel-ordered radius thresholds (per-point radius) or nullptr
     const float* xyz;      // the view's cloud (nullptr: SampleOpts::xyz)
 };
 
@@ -1236,7 +1262,7 @@ struct TileSmem {
     float2* cb;
     int* rowlo;
     int* rowbase;
-    float* rad;  // [cap] radii of the staged points (per-point radius)
+    float* rad;  // [cap] radius thresholds of the staged points (per-point radius)
     int* total;
     int* heavy;  // [kMaxTileRays] deferred rays (thread index in the tile), then their count
 };
@@ -1341,7 +1367,7 @@ __device__ __forceinline__ int stage_tile(const ViewArgs& view, const float* __r
             const float* p = xyz + 3 * (int64_t)id;
             sm.pts[i] = make_float4(p[0], p[1], p[2], __int_as_float(id));
         }
-        if (radius) sm.rad[i] = view.prad ? view.prad[gi] : radius[id];
+        if (radius) sm.rad[i] = view.prad ? view.prad[gi] : point_threshold(radius[id]);
     }
     for (int i = tid; i < RW * RH; i += nthr) {
         const int2 e = sm.hash[i];
@@ -1479,7 +1505,7 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
         if (!pts_in && pts) pts[e.x + j] = q;
         if (radius) {
             const float rj = radius[id];
-            if (prad) prad[e.x + j] = rj;
+            if (prad) prad[e.x + j] = point_threshold(rj);
             rb = j == 0 ? rj : fmaxf(rb, rj);
         }
         const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
