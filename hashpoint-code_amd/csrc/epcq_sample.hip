@@ -455,6 +455,7 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     o.feat = feat;
     o.feat_bf16 = feat_bf16;
     o.F = feat ? F : 0;
+    o.feat_vec = feat && F % 4 == 0 && reinterpret_cast<uintptr_t>(feat) % (feat_bf16 ? 8 : 16) == 0;
     SampleOut out{};
     out.s_max = s_max;
     out.n_samples = n_samples;

@@ -59,6 +59,7 @@ struct SampleOpts {
     const float* xyz;
     const void* feat;
     int feat_bf16, F;
+    int feat_vec;         // F % 4 == 0 and feat aligned for 4-feature loads
     int prefilter;        // cross-product rejection before the exact test
     int cull;             // skip neighbour buckets by their bounding spheres
     int heavy;            // a ray whose hit lanes hold more candidates is run by its whole wave (0: never)
@@ -596,19 +597,38 @@ __device__ __forceinline__ void write_neighbours(const SampleOpts& o, const Samp
                 if (b < nkeep) {
                     const float wb = inv[b] / den;
                     const int64_t base = (int64_t)nb.bi[b] * o.F + f0;
+                    float fv[4];
+                    if (o.feat_vec) {  // four features in one load
+                        if (o.feat_bf16) {
+                            const uint2 h = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(o.feat) + base);
+                            fv[0] = __uint_as_float(h.x << 16);
+                            fv[1] = __uint_as_float(h.x & 0xffff0000u);
+                            fv[2] = __uint_as_float(h.y << 16);
+                            fv[3] = __uint_as_float(h.y & 0xffff0000u);
+                        } else {
+                            const float4 q = *reinterpret_cast<const float4*>(static_cast<const float*>(o.feat) + base);
+                            fv[0] = q.x;
+                            fv[1] = q.y;
+                            fv[2] = q.z;
+                            fv[3] = q.w;
+                        }
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        if (f0 + c < o.F) {
-                            float fv;
-                            if (o.feat_bf16) {
-                                const uint16_t hb = static_cast<const uint16_t*>(o.feat)[base + c];
-                                fv = __uint_as_float((uint32_t)hb << 16);
-                            } else {
-                                fv = static_cast<const float*>(o.feat)[base + c];
+                        for (int c = 0; c < 4; ++c) {
+                            fv[c] = 0.0f;
+                            if (f0 + c < o.F) {
+                                if (o.feat_bf16) {
+                                    const uint16_t hb = static_cast<const uint16_t*>(o.feat)[base + c];
+                                    fv[c] = __uint_as_float((uint32_t)hb << 16);
+                                } else {
+                                    fv[c] = static_cast<const float*>(o.feat)[base + c];
+                                }
                             }
-                            acc[c] = acc[c] + wb * fv;
                         }
                     }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (f0 + c < o.F) acc[c] = acc[c] + wb * fv[c];
                 }
             }
 #pragma unroll
