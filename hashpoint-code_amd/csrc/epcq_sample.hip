@@ -117,15 +117,36 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     }
 }
 
+// Pixel-ordered copy of the points, one thread per list position: (x, y,
+// z, index bits) and, with per-point radii, the radius's squared threshold.
+// Positions holding no valid index (the -1 tail) are left alone; the hash
+// entries never point there.
+__global__ __launch_bounds__(kBlock) void point_copy_kernel(const int* __restrict__ sorted_idx,
+                                                            const float* __restrict__ xyz,
+                                                            const float* __restrict__ radius, int64_t n,
+                                                            float4* __restrict__ pts, float* __restrict__ prad) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int id = sorted_idx[i];
+    if (id < 0 || id >= n) return;
+    const float* q = xyz + 3 * (int64_t)id;
+    pts[i] = make_float4(q[0], q[1], q[2], __int_as_float(id));
+    if (prad) prad[i] = point_threshold(radius[id]);
+}
+
+// Bucket cones, one thread per pixel: from the pixel-ordered copy pts_in
+// when given, else gathering through sorted_idx / xyz (and then writing the
+// copy into pts, when not null).
 __global__ __launch_bounds__(kBlock) void bucket_cone_kernel(const int2* __restrict__ hash,
                                                              const int* __restrict__ sorted_idx,
                                                              const float* __restrict__ xyz,
+                                                             const float4* __restrict__ pts_in,
                                                              const float* __restrict__ radius, float r0,
                                                              const float* __restrict__ c2w, int64_t P,
                                                              float4* __restrict__ cone_a, float2* __restrict__ cone_b,
                                                              float4* __restrict__ pts, float* __restrict__ prad) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, nullptr, radius, r0, c2w, p, cone_a, cone_b, pts, prad);
+    if (p < P) bucket_cone_body(hash, sorted_idx, xyz, pts_in, radius, r0, c2w, p, cone_a, cone_b, pts, prad);
 }
 
 // hashtable_query_for_nearby_point (:222-283): one lane per ray, hits of each
@@ -269,8 +290,19 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     float4* pts = workspace_bytes >= rad_off ? reinterpret_cast<float4*>(w + pts_off) : nullptr;
     float* prad = o.radius && workspace_bytes >= rad_off + (size_t)n * 4 ? reinterpret_cast<float*>(w + rad_off)
                                                                         : nullptr;
-    bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
-        reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, o.radius, o.r0, cam->c2w, P, ca, cb, pts, prad);
+    if (pts) {
+        // the copy first, in parallel over list positions, then the cones from it
+        if (n > 0)
+            point_copy_kernel<<<dim3((unsigned)ceil_div(n, kBlock)), dim3(kBlock), 0, st>>>(sorted_idx, o.xyz,
+                                                                                          o.radius, n, pts, prad);
+        bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
+            reinterpret_cast<const int2*>(hash), nullptr, nullptr, pts, o.radius, o.r0, cam->c2w, P, ca, cb, nullptr,
+            nullptr);
+    } else {
+        bucket_cone_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(
+            reinterpret_cast<const int2*>(hash), sorted_idx, o.xyz, nullptr, o.radius, o.r0, cam->c2w, P, ca, cb,
+            nullptr, nullptr);
+    }
     o.cull = 1;
     view.cone_a = ca;
     view.cone_b = cb;
