@@ -52,7 +52,13 @@ def test_squared_threshold_is_exact():
         for e2 in np.concatenate([xs, ys]):
             assert (e2 < s) == (np.sqrt(e2) < r32)
     assert C.squared_threshold(0.0) == 0.0 and C.squared_threshold(-1.0) == 0.0
+    assert C.squared_threshold(float("nan")) == 0.0  # no distance is below NaN
     assert C.squared_threshold(float("inf")) == float("inf")
+    assert C.squared_threshold(1e30) == float("inf")  # every finite distance is below it
+    # radii whose square underflows: only the distance 0 is below them
+    for r in [1e-23, 1e-40]:
+        s = np.float32(C.squared_threshold(r))
+        assert s == np.nextafter(np.float32(0), np.float32(1)) and np.sqrt(s) >= np.float32(r)
 
 
 def test_max_live_samples_bound():
