@@ -135,8 +135,11 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
                                                                  const BatchView* __restrict__ views,
                                                                  const int* __restrict__ prefix,
                                                                  const float* __restrict__ radius, float r0) {
-    __shared__ int s_mem[2 * kMaxFineTile];
+    // placement's pixel counts and cursors (2 * kMaxFineTile ints), then the
+    // bin's pixel-ordered points for the cones when they fit
+    __shared__ float4 s_buf[kMaxFineTile / 2];
     __shared__ int s_wave[kBuildWaves];
+    int* s_mem = reinterpret_cast<int*>(s_buf);
     const int n_items = h->items[kFine], V = h->V;
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int v = find_view(prefix, V, item);
@@ -144,8 +147,13 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         const int b = item - prefix[v];
         fine_body(bv.b, b, s_mem, s_wave);
         const int bs = bv.b.bstart[b], tot = bv.b.btotal[b];
-        // fine_body wrote the pixel-ordered point copy from the binned records
+        // fine_body wrote the pixel-ordered point copy from the binned records;
+        // the cones read it from LDS (one coalesced copy instead of each
+        // thread's serial walks through L2) when the bin fits
         const float4* pts = bv.view.pts;
+        const bool staged = tot <= kMaxFineTile / 2;
+        if (staged)
+            for (int i = threadIdx.x; i < tot; i += kBuildBlock) s_buf[i] = pts[bs + i];
         if (radius) {
             float* prad = const_cast<float*>(bv.view.prad);
             for (int i = bs + (int)threadIdx.x; i < bs + tot; i += kBuildBlock)
@@ -153,12 +161,17 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         }
         __syncthreads();
         const int64_t p0 = (int64_t)b << bv.b.ts;
+        float4* cone_a = const_cast<float4*>(bv.view.cone_a);
+        float2* cone_b = const_cast<float2*>(bv.view.cone_b);
         for (int f = threadIdx.x; f < (1 << bv.b.ts); f += kBuildBlock) {
             const int64_t p = p0 + f;
-            if (p < bv.b.P)
-                bucket_cone_body(bv.view.hash, nullptr, nullptr, pts, radius, r0, bv.view.c2w, p,
-                                 const_cast<float4*>(bv.view.cone_a), const_cast<float2*>(bv.view.cone_b), nullptr,
-                                 nullptr);
+            if (p >= bv.b.P) continue;
+            if (staged)
+                bucket_cone_body(bv.view.hash, nullptr, nullptr, s_buf, radius, r0, bv.view.c2w, p, cone_a, cone_b,
+                                 nullptr, nullptr, bs);
+            else
+                bucket_cone_body(bv.view.hash, nullptr, nullptr, pts, radius, r0, bv.view.c2w, p, cone_a, cone_b,
+                                 nullptr, nullptr);
         }
         __syncthreads();
     }

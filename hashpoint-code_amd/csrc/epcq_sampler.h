@@ -1490,27 +1490,29 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
 }
 
 // Cone of pixel p's points seen from the view origin o (see cone_gap).  The
-// points come from pts_in (pixel-ordered (x, y, z, index bits)) when given,
-// else through sorted_idx / xyz, in which case they are also copied in list
-// order into pts (when not null); with per-point radii, the radii are copied
-// in list order into prad (when not null).  One thread per pixel; two passes over the
-// bucket: the axis (the normalised sum of the unit directions, each oriented
-// to the first one's side, since cones bound lines), then the largest angle
-// to it.  A bucket holding a point at o or a non-finite coordinate gets a cone
-// that never culls.
+// points come from pts_in (pixel-ordered (x, y, z, index bits), list position
+// i at pts_in[i - pts_base]) when given, else through sorted_idx / xyz, in
+// which case they are also copied in list order into pts (when not null);
+// with per-point radii, the radii are copied in list order into prad (when
+// not null).  One thread per pixel; two passes over the bucket: the axis
+// (the normalised sum of the unit directions, each oriented to the first
+// one's side, since cones bound lines), then the largest angle to it.  A
+// bucket holding a point at o or a non-finite coordinate gets a cone that
+// never culls.
 __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, const int* __restrict__ sorted_idx,
                                                  const float* __restrict__ xyz, const float4* __restrict__ pts_in,
                                                  const float* __restrict__ radius, float r0,
                                                  const float* __restrict__ c2w, int64_t p,
                                                  float4* __restrict__ cone_a, float2* __restrict__ cone_b,
-                                                 float4* __restrict__ pts, float* __restrict__ prad) {
+                                                 float4* __restrict__ pts, float* __restrict__ prad,
+                                                 int pts_base = 0) {
     const int2 e = hash[p];
     const float never = __builtin_inff();
     // an empty pixel's cone is never read (every reader tests the bucket's
     // count first), so it is not written
     if (e.y <= 0) return;
     auto point = [&](int j) {
-        if (pts_in) return pts_in[e.x + j];
+        if (pts_in) return pts_in[e.x + j - pts_base];
         const int id = sorted_idx[e.x + j];
         const float* q = xyz + 3 * (int64_t)id;
         return make_float4(q[0], q[1], q[2], __int_as_float(id));
@@ -1558,16 +1560,19 @@ __device__ __forceinline__ void bucket_cone_body(const int2* __restrict__ hash, 
         return;
     }
     const float a0 = s0 / sn, a1 = s1 / sn, a2 = s2 / sn;
-    float cmin = 1.0f, smax = 0.0f;
+    // the largest sine is the square root of the largest squared one (the
+    // rounded square root is monotonic), so one root per bucket
+    float cmin = 1.0f, s2max = 0.0f;
     for (int j = 0; j < e.y; ++j) {
-        const float4 q = pts_in ? pts_in[e.x + j] : (pts ? pts[e.x + j] : point(j));
+        const float4 q = pts_in ? pts_in[e.x + j - pts_base] : (pts ? pts[e.x + j] : point(j));
         const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
         const float inv = __builtin_amdgcn_rsqf(sqnorm3(w0, w1, w2));
         const float u0 = w0 * inv, u1 = w1 * inv, u2 = w2 * inv;
         cmin = fminf(cmin, fabsf(dot3(u0, u1, u2, a0, a1, a2)));
         const float x0 = u1 * a2 - u2 * a1, x1 = u2 * a0 - u0 * a2, x2 = u0 * a1 - u1 * a0;
-        smax = fmaxf(smax, sqrtf(sqnorm3(x0, x1, x2)));
+        s2max = fmaxf(s2max, sqnorm3(x0, x1, x2));
     }
+    const float smax = sqrtf(s2max);
     // rho, cos phi rounded down, sin phi up: each carries a few ulp of
     // error (reciprocal square root, products, sums), well inside 2e-6
     const float rl = rho * (1.0f - 2e-6f);
