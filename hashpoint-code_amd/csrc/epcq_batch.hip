@@ -129,6 +129,94 @@ __global__ __launch_bounds__(kBuildBlock) void batch_bin_kernel(const BatchHeade
     }
 }
 
+// Cone of pixel p (as bucket_cone_body) computed by a group of G = 2^gs
+// adjacent lanes, each taking every G-th point of the bucket from s_pts (list
+// position i at s_pts[i - base]); for bins of fewer pixels than threads.  The
+// orientation reference is the bucket's first point, as in the one-thread
+// form; sums meet in a butterfly (every lane ends with the same bits), minima
+// and maxima are exact in any order.  Every lane of the wave calls this
+// (shuffles); `live` is false for lanes without a pixel.
+__device__ __forceinline__ void bucket_cone_group(const int2* __restrict__ hash, const float4* s_pts, int base,
+                                                  const float* __restrict__ radius, float r0,
+                                                  const float* __restrict__ c2w, int64_t p, bool live, int gs,
+                                                  float4* __restrict__ cone_a, float2* __restrict__ cone_b) {
+    const int G = 1 << gs, sub = __lane_id() & (G - 1);
+    const int2 e = live ? hash[p] : make_int2(0, 0);
+    const float never = __builtin_inff();
+    const float o0 = c2w[3], o1 = c2w[7], o2 = c2w[11];
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
+    if (e.y > 0) {
+        const float4 q = s_pts[e.x - base];
+        const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
+        const float x = sqnorm3(w0, w1, w2);
+        if (x > 1e-30f && x < never) {
+            const float inv = __builtin_amdgcn_rsqf(x);
+            f0 = w0 * inv;
+            f1 = w1 * inv;
+            f2 = w2 * inv;
+        }
+    }
+    float rho = never, rb = radius ? __builtin_nanf("") : r0;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+    int bad = 0;
+    for (int j = sub; j < e.y; j += G) {
+        const float4 q = s_pts[e.x + j - base];
+        if (radius) rb = fmaxf(rb, radius[__float_as_int(q.w)]);
+        const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
+        const float x = sqnorm3(w0, w1, w2);
+        if (!(x > 1e-30f && x < never)) {
+            bad = 1;
+            continue;
+        }
+        const float inv = __builtin_amdgcn_rsqf(x);
+        const float n = x * inv;
+        const float u0 = w0 * inv, u1 = w1 * inv, u2 = w2 * inv;
+        const float sg = dot3(u0, u1, u2, f0, f1, f2) < 0.0f ? -1.0f : 1.0f;
+        s0 = s0 + sg * u0;
+        s1 = s1 + sg * u1;
+        s2 = s2 + sg * u2;
+        rho = fminf(rho, n);
+    }
+    for (int off = 1; off < G; off <<= 1) {
+        s0 = s0 + __shfl_xor(s0, off);
+        s1 = s1 + __shfl_xor(s1, off);
+        s2 = s2 + __shfl_xor(s2, off);
+        rho = fminf(rho, __shfl_xor(rho, off));
+        rb = fmaxf(rb, __shfl_xor(rb, off));
+        bad |= __shfl_xor(bad, off);
+    }
+    const float sn = sqrtf(sqnorm3(s0, s1, s2));
+    const bool never_culls = bad || !(sn > 0.0f) || !(rb <= never);
+    const float a0 = never_culls ? 1.0f : s0 / sn, a1 = never_culls ? 0.0f : s1 / sn,
+                a2 = never_culls ? 0.0f : s2 / sn;
+    float cmin = 1.0f, s2max = 0.0f;
+    if (!never_culls) {
+        for (int j = sub; j < e.y; j += G) {
+            const float4 q = s_pts[e.x + j - base];
+            const float w0 = q.x - o0, w1 = q.y - o1, w2 = q.z - o2;
+            const float inv = __builtin_amdgcn_rsqf(sqnorm3(w0, w1, w2));
+            const float u0 = w0 * inv, u1 = w1 * inv, u2 = w2 * inv;
+            cmin = fminf(cmin, fabsf(dot3(u0, u1, u2, a0, a1, a2)));
+            const float x0 = u1 * a2 - u2 * a1, x1 = u2 * a0 - u0 * a2, x2 = u0 * a1 - u1 * a0;
+            s2max = fmaxf(s2max, sqnorm3(x0, x1, x2));
+        }
+    }
+    for (int off = 1; off < G; off <<= 1) {
+        cmin = fminf(cmin, __shfl_xor(cmin, off));
+        s2max = fmaxf(s2max, __shfl_xor(s2max, off));
+    }
+    if (e.y <= 0 || sub != 0) return;
+    if (never_culls) {
+        cone_a[p] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        cone_b[p] = make_float2(0.0f, never);
+        return;
+    }
+    const float smax = sqrtf(s2max);
+    const float rl = rho * (1.0f - 2e-6f);
+    cone_a[p] = make_float4(a0, a1, a2, rl * (cmin - 2e-6f));
+    cone_b[p] = make_float2(rl * (smax + 2e-6f), rb * (1.0f + 1e-6f) + 4e-6f * rho);
+}
+
 // fine, then for the same bin: the pixel-ordered copy of its points and the
 // cones of its pixels (the samplers' bucket bounds), while they are in cache.
 __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHeader* __restrict__ h,
@@ -163,7 +251,16 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
         const int64_t p0 = (int64_t)b << bv.b.ts;
         float4* cone_a = const_cast<float4*>(bv.view.cone_a);
         float2* cone_b = const_cast<float2*>(bv.view.cone_b);
-        for (int f = threadIdx.x; f < (1 << bv.b.ts); f += kBuildBlock) {
+        // bins of fewer pixels than threads give each pixel 2^gs threads (at
+        // most 8); gs is uniform over the block, so whole waves take the path
+        int gs = 0;
+        while (gs < 3 && (kBuildBlock >> (bv.b.ts + gs + 1)) > 0) ++gs;
+        if (staged && gs > 0) {
+            const int f = (int)threadIdx.x >> gs;
+            const int64_t p = p0 + f;
+            bucket_cone_group(bv.view.hash, s_buf, bs, radius, r0, bv.view.c2w, p,
+                              f < (1 << bv.b.ts) && p < bv.b.P, gs, cone_a, cone_b);
+        } else for (int f = threadIdx.x; f < (1 << bv.b.ts); f += kBuildBlock) {
             const int64_t p = p0 + f;
             if (p >= bv.b.P) continue;
             if (staged)
