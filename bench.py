@@ -519,13 +519,27 @@ def bench_per_view(args, wl):
 
 
 # ----------------------------------------------------------------------------
-def cpu_baseline_views(wl, opt, target_s, mode, max_views=4):
+def cpu_model():
+    """The host CPU's model name (as lscpu reports it)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_views(wl, opt, target_s, mode, reps=5):
     """The oracle (a C++ restatement of the reference path, OpenMP) on the
-    host cores, over a bounded sample: the full hash build of up to
-    `max_views` views and the sampling of a central row block of each, grown
-    until the sample takes about `target_s`.  rays/s of the whole workload =
-    total rays / (mean build time x views + sampling time per ray x total
-    rays)."""
+    host cores, over a bounded, stratified sample: one view of each
+    resolution class of the workload; per view the full hash build and the
+    sampling of a fixed set of uniformly drawn random rays, each timed as one
+    warm-up and then the median of `reps` runs.  The ray count per view is
+    sized so the whole sample takes about `target_s`.  rays/s of the whole
+    workload = total rays / sum over classes of (views x build time + rays x
+    sampling time per ray)."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import oracle
@@ -535,37 +549,57 @@ def cpu_baseline_views(wl, opt, target_s, mode, max_views=4):
     oopt = oracle.options_from(opt)
     m = oracle.MODE_KNN if mode == "knn" else oracle.MODE_COMPAT
     cams = wl["cams"]
-    sub = cams[:max_views]
-    t_builds, t_samp, n_samp = [], 0.0, 0
-    budget = target_s / len(sub)
-    for cam in sub:
-        t0 = time.perf_counter()
+    classes = {}
+    for c in cams:
+        classes.setdefault((c.width, c.height), []).append(c)
+    rng = np.random.default_rng(12345)
+    per_class = []
+    budget = target_s / (len(classes) * (reps + 1))  # seconds per sampling run
+
+    def median_time(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    for (w, h), group in sorted(classes.items()):
+        cam = group[0]
+        t_build = median_time(lambda: oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width))
         oi, od, oh = oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width)
-        t_builds.append(time.perf_counter() - t0)
         ocam = oracle.camera_from(cam)
-        P = cam.width * cam.height
-        rows = max(1, cam.height // 64)
-        while True:
-            first = max(0, cam.height // 2 - rows // 2) * cam.width
-            rays = np.arange(first, min(P, first + rows * cam.width), dtype=np.int32)
-            t1 = time.perf_counter()
+        P = w * h
+        feat = np.zeros((len(pts), wl["F"]), np.float32) if wl["F"] else None
+
+        def sample(rays):
             res = oracle.quick_sampling(pts, ocam, oopt, oi, oh, radius=wl["radius"], mode=m, rays=rays, with_nb=True)
-            if wl["F"]:
+            if feat is not None:
                 live = res[3].reshape(-1)
-                feat = np.zeros((len(pts), wl["F"]), np.float32)
                 oracle.idw(pts, feat, res[0].reshape(-1, 3)[live], res[1].reshape(-1, res[1].shape[-1])[live])
-            dt = time.perf_counter() - t1
-            if dt >= budget / 4 or rows >= cam.height:
-                t_samp += dt
-                n_samp += rays.size
-                break
-            rows = min(cam.height, rows * 4)
+
+        # size the ray set from a probe run, then time it
+        probe = rng.integers(0, P, size=min(P, 2000)).astype(np.int32)
+        t0 = time.perf_counter()
+        sample(probe)
+        per_ray = max((time.perf_counter() - t0) / probe.size, 1e-9)
+        count = int(min(P, max(2000, budget / per_ray)))
+        rays = rng.integers(0, P, size=count).astype(np.int32)
+        t_samp = median_time(lambda: sample(rays))
+        per_class.append({"resolution": f"{w}x{h}", "views": len(group), "build_s": t_build,
+                          "rays_timed": count, "sample_s": t_samp, "rays_per_view": P})
     total = sum(c.width * c.height for c in cams)
-    rate = total / (float(np.mean(t_builds)) * len(cams) + t_samp / n_samp * total)
-    return {"value": rate, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sub)} of {len(cams)} views: full hash build (mean {np.mean(t_builds):.3f} s) + "
-                      f"sampling of {n_samp} central rays ({t_samp:.2f} s), {mode} mode, scaled to "
-                      f"{total} rays; oracle with {threads} OpenMP threads"}
+    denom = sum(pc["views"] * (pc["build_s"] + pc["rays_per_view"] * pc["sample_s"] / pc["rays_timed"])
+                for pc in per_class)
+    rate = total / denom
+    desc = "; ".join(f'{pc["resolution"]}: build {pc["build_s"] * 1e3:.1f} ms, {pc["rays_timed"]} random rays '
+                     f'in {pc["sample_s"] * 1e3:.1f} ms' for pc in per_class)
+    return {"value": rate, "unit": "rays/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "method": f"one view per resolution class, 1 warm-up + median of {reps} runs per timing",
+            "sample": f"{desc}; {mode} mode, scaled to {total} rays over {len(cams)} views; oracle with "
+                      f"{threads} OpenMP threads",
+            "per_class": per_class}
 
 
 def calibrate_copy(mib):

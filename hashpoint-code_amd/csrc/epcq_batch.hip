@@ -27,51 +27,21 @@
 #include <numeric>
 #include <vector>
 
-#include "epcq_hashbuild.h"
-#include "epcq_sampler.h"
+#include "epcq_batch_kernels.h"
+
+EPCQ_EXTERN_BATCH(EPCQ_MODE_COMPAT, epcq::kCompatBuffer)
+EPCQ_EXTERN_BATCH(EPCQ_MODE_KNN, 8)
+EPCQ_EXTERN_BATCH(EPCQ_MODE_KNN, 16)
+EPCQ_EXTERN_BATCH(EPCQ_MODE_KNN, 32)
 
 extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
 
 namespace epcq {
 
 float squared_threshold(float r);
+int check_tuning(const epcq_tuning& t, const char* who);
 
 namespace {
-
-constexpr int kMaxBins = 1 << kMaxBinBits;      // LDS cursors of count / bin
-constexpr int kMaxFineTile = 1 << 12;          // pixels per bin (fine LDS)
-constexpr int kStages = 6;                     // item spaces: count, col, scan, fine, pixels, tiles
-enum { kCount = 0, kCol = 1, kScan = 2, kFine = 3, kTot = 4, kTile = 5 };
-
-struct BatchView {
-    BuildView b;
-    ViewArgs view;
-    float c2w[12];
-    float w2c[12];
-    float K[9];
-    int tiles_x, tiles;
-    int col_groups;
-    int64_t ray_off;
-    int64_t pix_off;
-};
-
-struct BatchHeader {
-    int V;
-    int items[kStages];
-    int counter;    // next sample item
-    SampleOut out;  // compact outputs of the current views (moves with the view set)
-};
-
-// Largest v with prefix[v] <= item (prefix has V + 1 entries, prefix[0] = 0).
-__device__ __forceinline__ int find_view(const int* __restrict__ prefix, int V, int item) {
-    int lo = 0, hi = V;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= item) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __global__ void batch_reset_kernel(BatchHeader* h) { h->counter = 0; }
 
@@ -274,38 +244,6 @@ __global__ __launch_bounds__(kBuildBlock) void batch_fine_kernel(const BatchHead
     }
 }
 
-template <int MODE, int KMAX, int SMAX, bool PR>
-__global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restrict__ h,
-                                                           const BatchView* __restrict__ views,
-                                                           const int* __restrict__ prefix, SampleOpts o,
-                                                           TilePlan tp) {
-    extern __shared__ float4 s_dyn[];
-    __shared__ int s_total;
-    __shared__ int s_item;
-    __shared__ int s_heavy[kMaxTileRays + 1];
-    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
-    const int n_items = h->items[kTile], V = h->V;
-    const SampleOut out = h->out;
-    while (true) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&h->counter, 1);
-        __syncthreads();
-        const int item = s_item;
-        __syncthreads();
-        if (item >= n_items) break;
-        const int v = find_view(prefix, V, item);
-        const BatchView& bv = views[v];
-        // a tile whose halo region exceeds the LDS reads global memory right
-        // away: the densest views come first, so these long tiles overlap
-        // with the rest of the batch
-        const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp,
-                                                                                item - prefix[v], bv.tiles_x,
-                                                                                bv.ray_off, sm);
-        if (!staged)
-            sample_tile_global<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp, item - prefix[v],
-                                                                  bv.tiles_x, bv.ray_off, sm);
-    }
-}
-
 }  // namespace
 }  // namespace epcq
 
@@ -316,7 +254,6 @@ struct epcq_batch {
     epcq_options opt;
     int mode, s_max, bound;
     TilePlan tp;
-    int prefilter;
     int heavy;
     int max_views;
     char* arena;
@@ -402,20 +339,14 @@ int check_views(const epcq_view_desc* views, int n_views) {
 
 template <int MODE, int KMAX, int SMAX>
 void enqueue_sample(epcq_batch* b, const SampleOpts& o, hipStream_t st) {
-    const BatchView* dv = b->d_views;
     const int* pt = b->d_prefix + kTile * (b->max_views + 1);  // tiles, densest views first
-    if (o.radius)
-        batch_sample_kernel<MODE, KMAX, SMAX, true>
-            <<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(b->d_header, dv, pt, o, b->tp);
-    else
-        batch_sample_kernel<MODE, KMAX, SMAX, false>
-            <<<dim3(b->grid_sample), dim3(b->tp.tw * b->tp.th), b->tp.lds_bytes, st>>>(b->d_header, dv, pt, o, b->tp);
+    launch_batch_sample<MODE, KMAX, SMAX>(b->grid_sample, b->tp, b->d_header, b->d_views, pt, o, st);
 }
 
 template <int MODE, int KMAX>
 void enqueue_sample_smax(epcq_batch* b, const SampleOpts& o, hipStream_t st) {
     if (b->bound <= 2) enqueue_sample<MODE, KMAX, 2>(b, o, st);
-    else enqueue_sample<MODE, KMAX, 16>(b, o, st);
+    else enqueue_sample<MODE, KMAX, kRegSamples>(b, o, st);
 }
 
 // Enqueues the pass; ev (optional, 4 events) brackets the stages
@@ -427,7 +358,7 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     const BatchView* dv = b->d_views;
     int* pf = b->d_prefix;
     const dim3 blk(kBuildBlock), gb(b->grid_build);
-    if (ev) hipEventRecord(ev[0], st);
+    if (ev) (void)hipEventRecord(ev[0], st);
     batch_reset_kernel<<<1, 1, 0, st>>>(h);
     batch_count_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 0);
@@ -435,9 +366,9 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     batch_binscan_kernel<<<gb, blk, 0, st>>>(h, dv);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 1);
     batch_bin_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
-    if (ev) hipEventRecord(ev[1], st);
+    if (ev) (void)hipEventRecord(ev[1], st);
     batch_fine_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kFine * M, b->radius, b->opt.radius_threshold);
-    if (ev) hipEventRecord(ev[2], st);
+    if (ev) (void)hipEventRecord(ev[2], st);
     SampleOpts o{};
     o.k = b->opt.kernel_size;
     o.ko = (b->opt.kernel_size + 1) / 2;
@@ -452,14 +383,13 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     o.S = b->opt.num_sample_points_per_ray;
     o.Kc = b->opt.num_point_cloud_per_sp;
     o.xyz = nullptr;  // each view's cloud comes from its descriptor
-    o.prefilter = b->prefilter;
     o.cull = 1;
     o.heavy = b->heavy;
     if (b->mode == EPCQ_MODE_COMPAT) enqueue_sample_smax<EPCQ_MODE_COMPAT, kCompatBuffer>(b, o, st);
     else if (o.Kc <= 8) enqueue_sample_smax<EPCQ_MODE_KNN, 8>(b, o, st);
     else if (o.Kc <= 16) enqueue_sample_smax<EPCQ_MODE_KNN, 16>(b, o, st);
     else enqueue_sample_smax<EPCQ_MODE_KNN, 32>(b, o, st);
-    if (ev) hipEventRecord(ev[3], st);
+    if (ev) (void)hipEventRecord(ev[3], st);
     return check_launch("batch");
 }
 
@@ -509,7 +439,8 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     const float rm = radius ? r_max : opt->radius_threshold;
     int bound = std::min<int>(epcq_max_live_samples(opt, rm), s_max);
     if (radius) bound = s_max;
-    if (bound > 16) return fail(EPCQ_ERR_INVALID, "batch_create: more than 16 samples per ray");
+    if (tuning)
+        if (int e = check_tuning(*tuning, "batch_create")) return e;
     epcq_batch* b = new (std::nothrow) epcq_batch();
     if (!b) return fail(EPCQ_ERR_INVALID, "batch_create: out of host memory");
     b->opt = *opt;
@@ -521,7 +452,6 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     else epcq_get_tuning(&tu);
     b->tp = plan_tiles(opt->kernel_size, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags, radius != nullptr);
     b->tp.xcd = 0;
-    b->prefilter = (tu.flags & EPCQ_TUNE_PREFILTER) != 0;
     b->heavy = std::max(tu.heavy, 0);
     b->max_views = max_views;
     b->arena = static_cast<char*>(arena);
@@ -681,11 +611,11 @@ extern "C" int epcq_batch_capture(epcq_batch* b, void* stream) {
     hipStream_t st = as_stream(stream);
     if (!st) return fail(EPCQ_ERR_INVALID, "batch_capture: capture needs a non-default stream");
     if (b->exec) {
-        hipGraphExecDestroy(b->exec);
+        (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
     if (b->graph) {
-        hipGraphDestroy(b->graph);
+        (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -698,7 +628,7 @@ extern "C" int epcq_batch_capture(epcq_batch* b, void* stream) {
     hipGraphExec_t ex = nullptr;
     const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (e2 != hipSuccess) {
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         return fail(EPCQ_ERR_HIP, std::string("batch_capture/instantiate: ") + hipGetErrorString(e2));
     }
     b->graph = g;
@@ -723,14 +653,14 @@ extern "C" int epcq_batch_profile(epcq_batch* b, void* stream, float* stage_ms) 
         if (hipEventCreate(&ev[i]) != hipSuccess) return check_launch("batch_profile/event");
     int rc = enqueue_all(b, st, ev);
     if (rc == EPCQ_OK && hipEventSynchronize(ev[3]) != hipSuccess) rc = check_launch("batch_profile/sync");
-    for (int i = 0; rc == EPCQ_OK && i < 3; ++i) hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
-    for (int i = 0; i < 4; ++i) hipEventDestroy(ev[i]);
+    for (int i = 0; rc == EPCQ_OK && i < 3; ++i) (void)hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
+    for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]);
     return rc;
 }
 
 extern "C" void epcq_batch_destroy(epcq_batch* b) {
     if (!b) return;
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     delete b;
 }

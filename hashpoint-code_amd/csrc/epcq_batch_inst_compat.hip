@@ -1,0 +1,4 @@
+// Persistent batched sampling kernels of neighbour mode compat (see epcq_batch_kernels.h).
+#include "epcq_batch_kernels.h"
+
+EPCQ_INSTANTIATE_BATCH(EPCQ_MODE_COMPAT, epcq::kCompatBuffer)

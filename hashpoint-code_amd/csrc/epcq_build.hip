@@ -210,17 +210,19 @@ extern "C" size_t epcq_build_hashtable_workspace_size(int64_t n, int32_t height,
     return plan_build(n, height, width).bytes;
 }
 
-extern "C" int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2c, const float* K,
-                                    int32_t height, int32_t width, int32_t sort, int32_t* out_idx,
-                                    float* out_depth, int32_t* out_hash, void* workspace,
-                                    size_t workspace_bytes, void* stream) {
-    if (height <= 0 || width <= 0) return fail(EPCQ_ERR_INVALID, "build_hashtable: image size must be positive");
-    if (n < 0 || n >= (int64_t(1) << 31)) return fail(EPCQ_ERR_INVALID, "build_hashtable: point count out of range");
-    if ((int64_t)height * width >= (int64_t(1) << 31) / 2)
-        return fail(EPCQ_ERR_INVALID, "build_hashtable: image too large");
+namespace epcq {
+namespace {
+
+int build_hashtable_impl(const float* xyz, int64_t n, const float* w2c, const float* K, int32_t height, int32_t width,
+                         int32_t sort, int32_t* out_idx, float* out_depth, int32_t* out_hash, void* workspace,
+                         size_t workspace_bytes, void* stream, const char* who) {
+    const std::string w(who);
+    if (height <= 0 || width <= 0) return fail(EPCQ_ERR_INVALID, w + ": image size must be positive");
+    if (n < 0 || n >= (int64_t(1) << 31)) return fail(EPCQ_ERR_INVALID, w + ": point count out of range");
+    if ((int64_t)height * width >= (int64_t(1) << 31) / 2) return fail(EPCQ_ERR_INVALID, w + ": image too large");
     const BuildPlan bp = plan_build(n, height, width);
     if (workspace_bytes < bp.bytes || (bp.bytes && !workspace))
-        return fail(EPCQ_ERR_WORKSPACE, "build_hashtable: workspace too small");
+        return fail(EPCQ_ERR_WORKSPACE, w + ": workspace too small");
     hipStream_t st = as_stream(stream);
     if (n == 0) return fill_launch(out_hash, 2 * bp.P, -1, st);
     const BuildView bv = make_build_view(bp, xyz, n, w2c, K, height, width, sort, workspace, out_idx, out_depth,
@@ -241,6 +243,17 @@ extern "C" int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2
     const size_t lds_fine = (size_t)2 * (size_t(1) << bp.ts) * 4;
     build_fine_kernel<<<dim3(bp.nbins), dim3(kBlock), lds_fine, st>>>(bv);
     return check_launch("build_hashtable/fine");
+}
+
+}  // namespace
+}  // namespace epcq
+
+extern "C" int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2c, const float* K,
+                                    int32_t height, int32_t width, int32_t sort, int32_t* out_idx,
+                                    float* out_depth, int32_t* out_hash, void* workspace,
+                                    size_t workspace_bytes, void* stream) {
+    return build_hashtable_impl(xyz, n, w2c, K, height, width, sort, out_idx, out_depth, out_hash, workspace,
+                                workspace_bytes, stream, "build_hashtable");
 }
 
 extern "C" int epcq_scatter_hashtable(const int32_t* point_idx, const int32_t* keys, const float* depth,

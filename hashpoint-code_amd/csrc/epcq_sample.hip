@@ -22,6 +22,13 @@
 #include "epcq_common.h"
 #include "epcq_sampler.h"
 
+namespace epcq {
+// Defined in epcq_sample_kernels.h, instantiated in epcq_sample_inst_*.hip.
+template <int MODE, int KMAX, int SMAX>
+void launch_lay(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
+                const TilePlan& tp, hipStream_t st);
+}  // namespace epcq
+
 extern "C" int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
 
 namespace epcq {
@@ -57,10 +64,21 @@ TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags, bool poin
     return p;
 }
 
+int check_tuning(const epcq_tuning& t, const char* who) {
+    const std::string w(who);
+    const int rays = t.tile_w * t.tile_h;
+    if (t.tile_w < 1 || t.tile_w > 64 || (t.tile_w & (t.tile_w - 1)) || t.tile_h < 1 ||
+        (rays != 64 && rays != 128 && rays != 256))
+        return fail(EPCQ_ERR_INVALID, w + ": tile_w must be a power of two <= 64 and tile_w * tile_h 64, 128 or 256");
+    if (t.tile_cap < 0) return fail(EPCQ_ERR_INVALID, w + ": negative tile_cap");
+    if (t.heavy < 0) return fail(EPCQ_ERR_INVALID, w + ": negative heavy");
+    if (t.flags & ~EPCQ_TUNE_XCD_ORDER) return fail(EPCQ_ERR_INVALID, w + ": unknown flags");
+    return EPCQ_OK;
+}
+
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxLiveSamples = 16;  // samples per ray the sampler can emit
 
 std::mutex g_tuning_mu;
 epcq_tuning g_tuning{32, 8, 1024, 0, kHeavyCandidates};
@@ -68,53 +86,6 @@ epcq_tuning g_tuning{32, 8, 1024, 0, kHeavyCandidates};
 epcq_tuning current_tuning() {
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     return g_tuning;
-}
-
-template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
-__global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
-                                                             int tiles_x) {
-    extern __shared__ float4 s_dyn[];
-    __shared__ int s_total;
-    __shared__ int s_heavy[kMaxTileRays + 1];
-    const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
-    const int tile = tp.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    // a tile whose halo region exceeds the LDS reads global memory
-    if (!sample_tile_body<MODE, KMAX, SMAX, LAY, PR>(o, out, view, tp, tile, tiles_x, 0, sm))
-        sample_tile_global<MODE, KMAX, SMAX, LAY, PR>(o, out, view, tp, tile, tiles_x, 0, sm);
-}
-
-// One thread per listed ray; rays with many candidates are deferred and then
-// run one per wave (the block's waves share the queue).
-template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
-__global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, SampleOut out, ViewArgs view,
-                                                             const int* __restrict__ rays, int64_t R) {
-    __shared__ int s_heavy[kMaxTileRays + 1];
-    static_assert(kBlock <= kMaxTileRays, "queue holds one entry per thread");
-    if (threadIdx.x == 0) s_heavy[kMaxTileRays] = 0;
-    __syncthreads();
-    const SrcGlobal<PR> src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
-                            view.cone_b, view.W};
-    const int64_t r0 = (int64_t)blockIdx.x * kBlock;
-    const int64_t ri = r0 + threadIdx.x;
-    if (ri < R) {
-        const int64_t q = rays[ri];
-        const bool valid = q >= 0 && q < (int64_t)view.W * view.H;
-        const int u = valid ? (int)(q % view.W) : 0, v = valid ? (int)(q / view.W) : 0;
-        Ray ray;
-        view_ray(view, u, v, ray);
-        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, ri, valid, true))
-            defer_ray(s_heavy, (int)threadIdx.x);
-    }
-    __syncthreads();
-    const int n = s_heavy[kMaxTileRays];
-    for (int h = (int)threadIdx.x / kWave; h < n; h += kBlock / kWave) {
-        const int64_t rj = r0 + s_heavy[h];
-        const int64_t q = rays[rj];  // deferred rays are valid
-        const int u = (int)(q % view.W), v = (int)(q / view.W);
-        Ray ray;
-        view_ray(view, u, v, ray);
-        run_ray_wave<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H, rj);
-    }
 }
 
 // Pixel-ordered copy of the points, one thread per list position: (x, y,
@@ -261,9 +232,7 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     o.S = opt->num_sample_points_per_ray;
     o.Kc = opt->num_point_cloud_per_sp;
     o.xyz = xyz;
-    const epcq_tuning tu = current_tuning();
-    o.prefilter = (tu.flags & EPCQ_TUNE_PREFILTER) != 0;
-    o.heavy = tu.heavy;
+    o.heavy = current_tuning().heavy;
     return o;
 }
 
@@ -310,51 +279,16 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     view.prad = prad;
 }
 
-template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
-void launch_kernels(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
-                const TilePlan& tp, hipStream_t st) {
-    if (rays) {
-        sample_list_kernel<MODE, KMAX, SMAX, LAY, PR><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
-            o, out, view, rays, R);
-    } else {
-        const int tiles_x = (int)ceil_div(view.W, tp.tw);
-        const int tiles = tiles_x * (int)ceil_div(view.H, tp.th);
-        const dim3 block(tp.tw * tp.th);
-        sample_tile_kernel<MODE, KMAX, SMAX, LAY, PR><<<dim3(tiles), block, tp.lds_bytes, st>>>(o, out, view, tp,
-                                                                                              tiles_x);
-    }
-}
-
-// Per-point radius is a kernel property (PR); the dense drop-in layout has
-// none (quick_sampling takes a uniform radius).
-template <int MODE, int KMAX, int SMAX, int LAY>
-void launch_one(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
-                const TilePlan& tp, hipStream_t st) {
-    if constexpr (LAY != kLayDense) {
-        if (o.radius) {
-            launch_kernels<MODE, KMAX, SMAX, LAY, true>(o, out, view, rays, R, tp, st);
-            return;
-        }
-    }
-    launch_kernels<MODE, KMAX, SMAX, LAY, false>(o, out, view, rays, R, tp, st);
-}
-
-template <int MODE, int KMAX, int SMAX>
-void launch_lay(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
-                const TilePlan& tp, hipStream_t st) {
-    if (out.sp_xyz) launch_one<MODE, KMAX, SMAX, kLayDense>(o, out, view, rays, R, tp, st);
-    else if (o.feat) launch_one<MODE, KMAX, SMAX, kLayFeat>(o, out, view, rays, R, tp, st);
-    else launch_one<MODE, KMAX, SMAX, kLayCompact>(o, out, view, rays, R, tp, st);
-}
-
 template <int MODE, int KMAX>
 void launch_smax(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                  const TilePlan& tp, int bound, hipStream_t st) {
     if (bound <= 2) launch_lay<MODE, KMAX, 2>(o, out, view, rays, R, tp, st);
-    else launch_lay<MODE, KMAX, kMaxLiveSamples>(o, out, view, rays, R, tp, st);
+    else launch_lay<MODE, KMAX, kRegSamples>(o, out, view, rays, R, tp, st);
 }
 
-// `bound`: samples one ray can emit (<= kMaxLiveSamples, checked by callers).
+// `bound`: samples of one ray that are stored (kernels with a bound of 2 keep
+// two sample depths in registers; the others keep kRegSamples and re-read
+// later ones from the outputs).
 int launch_sample(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                   int mode, int bound, hipStream_t st, const char* who) {
     if (R == 0) return EPCQ_OK;
@@ -372,14 +306,6 @@ int launch_sample(const SampleOpts& o, const SampleOut& out, const ViewArgs& vie
     return check_launch(who);
 }
 
-int live_bound(const epcq_options* opt, float r_max, const char* who, int* bound) {
-    *bound = epcq_max_live_samples(opt, r_max);
-    if (*bound > kMaxLiveSamples)
-        return fail(EPCQ_ERR_INVALID, std::string(who) + ": these options allow more than " +
-                                          std::to_string(kMaxLiveSamples) + " samples per ray");
-    return EPCQ_OK;
-}
-
 }  // namespace
 }  // namespace epcq
 
@@ -387,13 +313,7 @@ using namespace epcq;
 
 extern "C" int epcq_set_tuning(const epcq_tuning* t) {
     if (!t) return fail(EPCQ_ERR_INVALID, "set_tuning: null");
-    const int rays = t->tile_w * t->tile_h;
-    if (t->tile_w < 1 || t->tile_w > 64 || (t->tile_w & (t->tile_w - 1)) || t->tile_h < 1 ||
-        (rays != 64 && rays != 128 && rays != 256))
-        return fail(EPCQ_ERR_INVALID,
-                    "set_tuning: tile_w must be a power of two <= 64 and tile_w * tile_h 64, 128 or 256");
-    if (t->tile_cap < 0) return fail(EPCQ_ERR_INVALID, "set_tuning: negative tile_cap");
-    if (t->heavy < 0) return fail(EPCQ_ERR_INVALID, "set_tuning: negative heavy");
+    if (int e = check_tuning(*t, "set_tuning")) return e;
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_tuning = *t;
     return EPCQ_OK;
@@ -445,8 +365,7 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
     if (mode == EPCQ_MODE_KNN && opt->num_point_cloud_per_sp > 32)
         return fail(EPCQ_ERR_INVALID, "quick_sampling: num_point_cloud_per_sp > 32 in KNN mode");
     if (opt->num_sample_points_per_ray == 0) return EPCQ_OK;
-    int bound;
-    if (int e = live_bound(opt, opt->radius_threshold, "quick_sampling", &bound)) return e;
+    const int bound = epcq_max_live_samples(opt, opt->radius_threshold);
     SampleOpts o = make_opts(xyz, opt, nullptr);
     SampleOut out{};
     out.sp_xyz = sp_xyz;
@@ -476,13 +395,7 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     if (feat && (F < 1 || !w || !agg)) return fail(EPCQ_ERR_INVALID, "knn_sampling: bad feature arguments");
     if (rays && R < 0) return fail(EPCQ_ERR_INVALID, "knn_sampling: negative ray count");
     // with per-point radii the caller's s_max stands for the bound
-    int bound = s_max;
-    if (!radius) {
-        if (int e = live_bound(opt, opt->radius_threshold, "knn_sampling", &bound)) return e;
-        bound = std::min(bound, s_max);
-    } else if (bound > kMaxLiveSamples) {
-        return fail(EPCQ_ERR_INVALID, "knn_sampling: s_max above 16 with per-point radius");
-    }
+    const int bound = radius ? s_max : std::min<int>(epcq_max_live_samples(opt, opt->radius_threshold), s_max);
     SampleOpts o = make_opts(xyz, opt, radius);
     o.feat = feat;
     o.feat_bf16 = feat_bf16;

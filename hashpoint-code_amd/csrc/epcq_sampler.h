@@ -43,6 +43,9 @@
 namespace epcq {
 
 constexpr int kSlotTable = 8;
+// Sample depths a ray keeps in registers for its neighbour walks (kernels with
+// a larger sample bound re-read the later ones from the outputs they wrote).
+constexpr int kRegSamples = 16;
 // Rays whose hit lanes hold more candidates than this are run by a whole wave
 // when at most kHeavyMinority rays of their wave are (or when they hold 4x as
 // many).
@@ -60,7 +63,6 @@ struct SampleOpts {
     const void* feat;
     int feat_bf16, F;
     int feat_vec;         // F % 4 == 0 and feat aligned for 4-feature loads
-    int prefilter;        // cross-product rejection before the exact test
     int cull;             // skip neighbour buckets by their bounding spheres
     int heavy;            // a ray whose hit lanes hold more candidates is run by its whole wave (0: never)
 };
@@ -191,23 +193,6 @@ __device__ __forceinline__ void ray_t_e2(const Ray& r, const float4& p, float& t
     e2 = sqnorm3(p.x - q0, p.y - q1, p.z - q2);
 }
 
-// Conservative rejection before the exact test: true only if the point is
-// certainly at distance >= r from the ray.  |w x d| / |d| is the exact
-// distance of o + w from the ray; its fused-multiply-add estimate differs by
-// at most ~4 eps |w|, and the canonical rounded distance of ray_t_e2 by at
-// most ~eps (18 |w| + 4 |o| + 1) (rounding of t, the foot point and the norm).
-// The margin 3e-6 (1 + |w|_1 + |o|_1) exceeds their sum by more than 2x, so a
-// point rejected here fails the exact test too.
-__device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, float w2, float r, float omargin) {
-    const float aw = fabsf(w0) + fabsf(w1) + fabsf(w2);
-    const float rm = __builtin_fmaf(3e-6f, aw, r + omargin);
-    const float c0 = __builtin_fmaf(w1, ray.d[2], -(w2 * ray.d[1]));
-    const float c1 = __builtin_fmaf(w2, ray.d[0], -(w0 * ray.d[2]));
-    const float c2 = __builtin_fmaf(w0, ray.d[1], -(w1 * ray.d[0]));
-    const float cs = __builtin_fmaf(c2, c2, __builtin_fmaf(c1, c1, c0 * c0));
-    return cs > rm * rm * ray.dd;
-}
-
 // Bucket culling by view cones.  Every ray of a view starts at the same
 // origin o, so a bucket is summarised by the cone of its points seen from o:
 // unit axis a, rho <= |p - o| and phi >= the angle between the line through p
@@ -218,8 +203,10 @@ __device__ __forceinline__ bool surely_far(const Ray& ray, float w0, float w1, f
 // towards that bound, and B = the bucket's largest radius plus 4e-6 rho; the
 // ray side's cos / sin errors (a few ulp of rho) are inside the 4e-6 rho, and
 // the rounded distances the exact tests compute differ from the exact
-// distance by at most ~eps (18 |p - o| + 4 |o| + 1) (see surely_far), which
-// the slack of 4e-6 rho over 18 eps rho and the origin margin cover for
+// distance by at most ~eps (18 |p - o| + 4 |o| + 1) (the rounding of t, of the
+// foot point o + t d and of the norm, with |d| = 1 up to an ulp), which the
+// slack of 4e-6 rho over 18 eps rho and the origin margin
+// 3e-6 (1 + |o|_1) cover for
 // every |p - o| >= rho.  So a bucket clears the ray by `margin` -- cone gap
 // (s P - c Q) / |d| - B > margin, with c = |d . a| and s = |d x a| -- only if
 // no point of the bucket passes the exact test.  cone_clears evaluates the
@@ -247,15 +234,6 @@ template <class Src>
 __device__ __forceinline__ float point_limit(const SampleOpts& o, const Src& src, int j, const float4& p) {
     if constexpr (Src::kPR) return src.rad(j, p);
     else return o.r2lim;
-}
-
-// A radius at least the one whose squared threshold is lim, for the
-// conservative prefilter (the hardware square root's ulp is inside its
-// margin).
-template <bool PR>
-__device__ __forceinline__ float radius_bound(const SampleOpts& o, float lim) {
-    if constexpr (PR) return __builtin_amdgcn_sqrtf(lim) * 1.000001f;
-    else return o.r0;
 }
 
 // Lane sets of one ray: bit l for window lane l (k*k lanes, at most 64).
@@ -367,9 +345,6 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
             const float lim = point_limit(o, src, e.x + j, p);
-            if (o.prefilter && surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2],
-                                          radius_bound<Src::kPR>(o, lim), ray.omargin))
-                continue;
             float t, e2;
             ray_t_e2(ray, p, t, e2);
             if (within(e2, lim)) {
@@ -688,6 +663,14 @@ __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut&
     }
 }
 
+// Depth of emitted sample q of ray ri as this thread wrote it to the outputs
+// (for samples beyond the ones kept in registers).
+template <int LAY>
+__device__ __forceinline__ float stored_sample_t(const SampleOpts& o, const SampleOut& out, int64_t ri, int q) {
+    if constexpr (LAY == kLayDense) return out.sp_t[ri * o.S + q];
+    else return out.sample_t[ri * out.s_max + q];
+}
+
 // Samples and neighbours of one ray; writes every output element of row ri.
 // SMAX bounds the samples one ray can emit (epcq_max_live_samples).
 // With may_defer, a ray whose hit lanes hold more than o.heavy candidates is
@@ -851,8 +834,10 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
                 T -= alpha;
             }
         }
-        // Step 4: neighbours of the emitted samples, NS per walk.
-        const int nstore = min(min(ns, SMAX), dense ? o.S : out.s_max);
+        // Step 4: neighbours of the emitted samples, NS per walk.  Kernels
+        // with a sample bound above 2 keep kRegSamples depths in registers
+        // and re-read later ones from the outputs this thread wrote.
+        const int nstore = min(SMAX < kRegSamples ? min(ns, SMAX) : ns, dense ? o.S : out.s_max);
         for (int g = 0; g < nstore; g += NS) {
             float pend[NS][3];
             NbBuf<MODE, KMAX> buf[NS];
@@ -864,6 +849,7 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
 #pragma unroll
                 for (int e = 0; e < SMAX; ++e)
                     if (e == g + q) tq = ts_arr[e];
+                if (SMAX >= kRegSamples && g + q >= SMAX && q < cnt) tq = stored_sample_t<LAY>(o, out, ri, g + q);
                 pend[q][0] = ray.o[0] + tq * ray.d[0];
                 pend[q][1] = ray.o[1] + tq * ray.d[1];
                 pend[q][2] = ray.o[2] + tq * ray.d[2];
@@ -1052,12 +1038,8 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                 if (j < ey) {
                     const float4 p = src.point(ex + j);
                     const float lim = point_limit(o, src, ex + j, p);
-                    if (!(o.prefilter &&
-                          surely_far(ray, p.x - ray.o[0], p.y - ray.o[1], p.z - ray.o[2],
-                                     radius_bound<Src::kPR>(o, lim), ray.omargin))) {
-                        ray_t_e2(ray, p, t, e2);
-                        hit = within(e2, lim);
-                    }
+                    ray_t_e2(ray, p, t, e2);
+                    hit = within(e2, lim);
                 }
                 const uint64_t hb = __ballot(hit);
                 const bool keep = hit && kept + __popcll(hb & lanemask_lt()) < kSampleLaneHits;
@@ -1148,7 +1130,7 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
         }
         // Step 4: neighbours of the emitted samples, NS per walk
         constexpr bool dense = LAY == kLayDense;
-        const int nstore = min(min(ns, SMAX), dense ? o.S : out.s_max);
+        const int nstore = min(SMAX < kRegSamples ? min(ns, SMAX) : ns, dense ? o.S : out.s_max);
         for (int g = 0; g < nstore; g += NS) {
             float pend[NS][3];
             NbBuf<MODE, KMAX> buf[NS];
@@ -1160,6 +1142,12 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
 #pragma unroll
                 for (int x = 0; x < SMAX; ++x)
                     if (x == g + q) tq = ts_arr[x];
+                if (SMAX >= kRegSamples && g + q >= SMAX && q < cnt) {
+                    // lane 0 wrote the sample; it reads it back for the wave
+                    float back = 0.0f;
+                    if (lane == 0) back = stored_sample_t<LAY>(o, out, ri, g + q);
+                    tq = lane_bcast(back, 0);
+                }
                 pend[q][0] = ray.o[0] + tq * ray.d[0];
                 pend[q][1] = ray.o[1] + tq * ray.d[1];
                 pend[q][2] = ray.o[2] + tq * ray.d[2];

@@ -1,11 +1,12 @@
 """Autotuner for the samplers' launch shape.
 
 The full-image samplers take a tile shape (tile_w x tile_h rays per block),
-a staging capacity (points of a tile's halo region copied to LDS) and two
-flags (XCD-contiguous tile order, cross-product pre-filter); the best choice
+a staging capacity (points of a tile's halo region copied to LDS), a flag
+(XCD-contiguous tile order) and the outlier threshold of the wave-wide form
+(`heavy`); the best choice
 depends on the point density per pixel and the image sizes.  tune() times a
 caller-supplied workload under candidate settings with HIP events and keeps
-the fastest, searching one knob at a time (shape, then capacity, then flags).
+the fastest, searching one knob at a time (shape, then capacity, then flags, then heavy).
 Results are cached per (device, key) in memory and, when EPCQ_TUNE_CACHE
 names a file, in that JSON file.
 """
@@ -18,7 +19,7 @@ from . import csrc as _C
 
 SHAPES = [(16, 16), (8, 8), (16, 8), (32, 8), (32, 4), (64, 2)]
 CAPS = [1024, 512, 2048, 3000]
-FLAGS = [0, _C.TUNE_PREFILTER, _C.TUNE_XCD_ORDER]
+FLAGS = [0, _C.TUNE_XCD_ORDER]
 HEAVY = [_C.HEAVY_DEFAULT, 64, 256, 0]
 DEFAULT = {"tile_w": 32, "tile_h": 8, "tile_cap": 1024, "flags": 0, "heavy": _C.HEAVY_DEFAULT}
 

@@ -49,13 +49,15 @@ class MultiViewSampler:
     """Owns an arena and a batch object of the HIP library."""
 
     def __init__(self, options, mode="compat", s_max=None, max_views=256, radius=None, tuning=None,
-                 device="cuda:0"):
+                 device="cuda:0", r_max=None):
         self.device = torch.device(device)
         self.options = options
         self.mode = mode
         self.opts = _options_struct(options)
         self.radius = radius
-        r_max = float(radius.max().item()) if radius is not None and radius.numel() else float(options.radius_threshold)
+        if r_max is None:
+            r_max = _C.radius_max(radius) if radius is not None else float(options.radius_threshold)
+        r_max = float(r_max)
         self.s_max = int(s_max) if s_max is not None else max(1, _C.max_live_samples(options, r_max))
         self.r_max = r_max
         self.max_views = int(max_views)

@@ -62,7 +62,6 @@ class _Tuning(ctypes.Structure):
 
 
 TUNE_XCD_ORDER = 1
-TUNE_PREFILTER = 2
 
 
 class _ViewDesc(ctypes.Structure):
@@ -476,8 +475,24 @@ def max_live_samples(options, r_max):
     return int(_lib.epcq_max_live_samples(ctypes.byref(_options_struct(options)), float(r_max)))
 
 
+_RADIUS_MAX = {}
+
+
+def radius_max(radius):
+    """Largest per-point radius, read from the device once per radius tensor
+    (keyed by storage, size and version, so an in-place change is seen)."""
+    key = (radius.data_ptr(), radius.numel(), radius._version, str(radius.device))
+    r = _RADIUS_MAX.get(key)
+    if r is None:
+        r = float(radius.max().item()) if radius.numel() else 0.0
+        if len(_RADIUS_MAX) >= 64:
+            _RADIUS_MAX.clear()
+        _RADIUS_MAX[key] = r
+    return r
+
+
 def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, rays=None, radius=None,
-                 mode="knn", s_max=None, feat=None, cull=True):
+                 mode="knn", s_max=None, feat=None, cull=True, r_max=None):
     """Compact K-nearest sampling.
 
     Same samples as quick_sampling, stored per ray.  Returns a dict with
@@ -487,7 +502,10 @@ def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, 
     (float32 or bfloat16 [N,F]) also w float32 [R,s_max,Kc] and agg float32
     [R,s_max,F] (inverse-distance weights and weighted feature sum).
     `rays` int32 pixel ids (default: all pixels); `radius` float32 [N]
-    per-point radius (default: options.radius_threshold)."""
+    per-point radius (default: options.radius_threshold).  s_max defaults to
+    max_live_samples(options, r_max); with per-point radii r_max (their
+    maximum) is read once per radius tensor unless given.  The call does not
+    synchronise with the device when s_max or r_max is given."""
     device = _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable)
     H, W = int(cam.height), int(cam.width)
     hashtable = _hashtable_view(hashtable, H, W)
@@ -505,10 +523,9 @@ def knn_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, 
         _check_same_device(device, ("radius", radius))
         if radius.numel() != xyz.size(0):
             raise RuntimeError("radius must have one entry per point")
-        r_max = float(radius.max().item()) if radius.numel() else 0.0
-    else:
-        r_max = float(options.radius_threshold)
     if s_max is None:
+        if r_max is None:
+            r_max = radius_max(radius) if radius is not None else float(options.radius_threshold)
         s_max = max(1, max_live_samples(options, r_max))
     Kc = int(options.num_point_cloud_per_sp)
     knn = _MODES[mode] == MODE_KNN

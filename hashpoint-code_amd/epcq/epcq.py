@@ -203,10 +203,10 @@ class NeuralPoints(nn.Module):
     def knn_sampling(self, cam: CameraSpec, opt: RenderOptions, sorted_point_idx_list: torch.Tensor,
                      hashtable: torch.Tensor, rays: Optional[torch.Tensor] = None,
                      radius: Optional[torch.Tensor] = None, mode: str = "knn", feat: Optional[torch.Tensor] = None,
-                     s_max: Optional[int] = None, cull: bool = True):
+                     s_max: Optional[int] = None, cull: bool = True, r_max: Optional[float] = None):
         """Compact K-nearest sampling (see epcq.csrc.knn_sampling)."""
         return _C.knn_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list, hashtable,
-                               rays=rays, radius=radius, mode=mode, s_max=s_max, feat=feat, cull=cull)
+                               rays=rays, radius=radius, mode=mode, s_max=s_max, feat=feat, cull=cull, r_max=r_max)
 
 
 class FeatureQuery(nn.Module):

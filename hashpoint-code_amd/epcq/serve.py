@@ -21,7 +21,10 @@ batch's outputs are copied out of the arena on the slot's stream (one copy
 per output array) before the slot can be reused; a request's result tensors
 are views of that copy, ready when `done` is true, and they equal the
 results of the same view sampled alone: every ray is computed independently
-of the other views in its batch.
+of the other views in its batch.  When poll() completes a request it records
+the caller's current stream on the result tensors, so their memory is not
+reused before work enqueued there has run; a caller that reads them on
+another stream records that stream itself (Tensor.record_stream).
 """
 import collections
 import time
@@ -153,8 +156,14 @@ class PointServer:
         for slot in self.slots:
             if slot.event is not None and slot.event.query():
                 now = self.clock()
+                consumer = torch.cuda.current_stream(self.device)
                 for r in slot.requests:
                     r.finished = now
+                    # the copies were made on the slot's stream; from here the
+                    # caller's stream reads them, so the allocator must wait
+                    # for that stream's work before reusing their memory
+                    for t in r.result.values():
+                        t.record_stream(consumer)
                 done.extend(slot.requests)
                 slot.requests, slot.event = [], None
         for slot in self.slots:

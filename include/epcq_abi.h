@@ -73,12 +73,12 @@ const char* epcq_last_error(void);
  * rays (tile_w * tile_h in {64, 128, 256}, tile_w a power of two <= 64);
  * tile_cap = points of the tile's halo region staged in LDS (clamped to the
  * 64 KB block limit).  flags: EPCQ_TUNE_XCD_ORDER gives each XCD a
- * contiguous run of tiles; EPCQ_TUNE_PREFILTER enables the cross-product
- * rejection before the exact point-to-ray test.  heavy: a ray whose
+ * contiguous run of tiles (other bits are rejected).  heavy: a ray whose
  * unculled window buckets hold more than this many points (more than 4x this
  * many when over 16 rays of its wave of 64 exceed it) is sampled by a whole
  * wave (64 lanes share its candidate walks) after the tile's other rays,
- * instead of by one lane; 0 never does.  Results are identical either way.  Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
+ * instead of by one lane; 0 never does.  Results are identical either way.
+ * Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
  * 32, 8, 1024, 0, 128.
  */
 typedef struct epcq_tuning {
@@ -89,7 +89,6 @@ typedef struct epcq_tuning {
     int32_t heavy;
 } epcq_tuning;
 #define EPCQ_TUNE_XCD_ORDER 1
-#define EPCQ_TUNE_PREFILTER 2
 int epcq_set_tuning(const epcq_tuning* t);
 int epcq_get_tuning(epcq_tuning* t);
 

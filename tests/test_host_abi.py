@@ -86,9 +86,11 @@ def test_tuning_roundtrip_and_validation():
     import epcq.csrc as C
     old = C.get_tuning()
     try:
-        C.set_tuning(32, 4, 1024, C.TUNE_PREFILTER, heavy=7)
-        assert C.get_tuning() == {"tile_w": 32, "tile_h": 4, "tile_cap": 1024, "flags": C.TUNE_PREFILTER,
+        C.set_tuning(32, 4, 1024, C.TUNE_XCD_ORDER, heavy=7)
+        assert C.get_tuning() == {"tile_w": 32, "tile_h": 4, "tile_cap": 1024, "flags": C.TUNE_XCD_ORDER,
                                   "heavy": 7}
+        with pytest.raises(RuntimeError, match="unknown flags"):
+            C.set_tuning(16, 16, 1024, flags=2)
         with pytest.raises(RuntimeError, match="heavy"):
             C.set_tuning(16, 16, 1024, heavy=-1)
         with pytest.raises(RuntimeError, match="tile_w"):
