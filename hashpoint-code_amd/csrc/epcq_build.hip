@@ -25,6 +25,9 @@
 //               off-image tail of the output is filled with -1.
 // The output order is therefore (pixel, point index), or (pixel, depth, point
 // index) with sort, independent of scheduling.
+#include <mutex>
+#include <string>
+
 #include "epcq_hashbuild.h"
 
 namespace epcq {
@@ -254,6 +257,62 @@ extern "C" int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2
                                     size_t workspace_bytes, void* stream) {
     return build_hashtable_impl(xyz, n, w2c, K, height, width, sort, out_idx, out_depth, out_hash, workspace,
                                 workspace_bytes, stream, "build_hashtable");
+}
+
+// ---- build_hashtable without allocation --------------------------------------
+// The workspace of epcq_build_hashtable_noalloc is this file-scope device
+// array, one copy per GPU for the life of the process.  Host code never
+// passes `scratch` itself to a kernel (on the host that name is a stand-in,
+// not a device address): it passes the address hipGetSymbolAddress returns
+// for the current device, looked up once per device and kept in
+// g_scratch_ptr.  Every build kernel writes each workspace word before it
+// reads it, so nothing needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[EPCQ_NOALLOC_SCRATCH_BYTES];
+
+namespace epcq {
+namespace {
+
+constexpr size_t kNoallocScratchBytes = EPCQ_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch array (looked up on its first use).
+int scratch_address(void** out, const char* who) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return check_launch(who);
+    if (dev < 0 || dev >= kMaxDevices) return fail(EPCQ_ERR_INVALID, std::string(who) + ": device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(EPCQ_ERR_HIP, std::string(who) + ": hipGetSymbolAddress: " + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return EPCQ_OK;
+}
+
+}  // namespace
+}  // namespace epcq
+
+extern "C" int epcq_build_hashtable_noalloc(const float* xyz, int64_t n, const float* w2c, const float* K,
+                                            int32_t height, int32_t width, int32_t sort, int32_t* out_idx,
+                                            float* out_depth, int32_t* out_hash, void* stream) {
+    const char* who = "build_hashtable_noalloc";
+    if (height > 0 && width > 0 && n >= 0 && n < (int64_t(1) << 31) &&
+        (int64_t)height * width < (int64_t(1) << 31) / 2) {
+        const size_t need = plan_build(n, height, width).bytes;
+        if (need > kNoallocScratchBytes)
+            return fail(EPCQ_ERR_WORKSPACE, std::string(who) + ": needs " + std::to_string(need) +
+                                                " bytes of scratch, more than its " +
+                                                std::to_string(kNoallocScratchBytes) + " (too many points for this image)");
+    }
+    void* ws = nullptr;
+    if (int e = scratch_address(&ws, who)) return e;
+    return build_hashtable_impl(xyz, n, w2c, K, height, width, sort, out_idx, out_depth, out_hash, ws,
+                                kNoallocScratchBytes, stream, who);
 }
 
 extern "C" int epcq_scatter_hashtable(const int32_t* point_idx, const int32_t* keys, const float* depth,

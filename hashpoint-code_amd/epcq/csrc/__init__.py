@@ -94,6 +94,7 @@ ABI_SYMBOLS = {
     "epcq_scatter": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P]),
     "epcq_build_hashtable_workspace_size": (_SZ, [_I64, _I32, _I32]),
     "epcq_build_hashtable": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "epcq_build_hashtable_noalloc": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P]),
     "epcq_quick_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
                                    _I32, _P, _P, _P, _P, _P, _SZ, _P]),
     "epcq_sampling_workspace_size": (_SZ, [_I64, _I32, _I32]),

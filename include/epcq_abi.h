@@ -134,6 +134,23 @@ int epcq_build_hashtable(const float* xyz, int64_t n, const float* w2c, const fl
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * build_hashtable_noalloc -- no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): epcq_build_hashtable with
+ * its workspace in a device array the library holds, EPCQ_NOALLOC_SCRATCH_BYTES
+ * on each GPU it runs on, for the life of the process.  Same outputs as
+ * epcq_build_hashtable; returns EPCQ_ERR_WORKSPACE when
+ * epcq_build_hashtable_workspace_size(n, height, width) exceeds that size.
+ * It allocates nothing.  All calls on one GPU share its array: calls that
+ * overlap on one GPU (different streams, or threads) race and must not be
+ * made.
+ */
+#define EPCQ_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int epcq_build_hashtable_noalloc(const float* xyz, int64_t n, const float* w2c, const float* K,
+                                 int32_t height, int32_t width, int32_t sort,
+                                 int32_t* out_idx, float* out_depth, int32_t* out_hash, void* stream);
+
+/*
  * quick_sampling -- replaces `quick_sampling` (epcq/csrc/epcq.cpp:7,
  * epcq/csrc/epcq_render_topk.cu:341-417).  hash [H*W,2]; writes every element
  * of sp_xyz [H,W,S,3], pc_idx [H,W,S,Kc], sp_t [H,W,S] and mask [H,W,S]

@@ -124,6 +124,9 @@ int main() {
     check(epcq_build_hashtable(nullptr, 1000, nullptr, nullptr, 8, 8, 0, nullptr, nullptr, nullptr, nullptr, need - 1,
                                nullptr) == EPCQ_ERR_WORKSPACE,
           "workspace too small");
+    check(epcq_build_hashtable_noalloc(nullptr, int64_t(40) << 20, nullptr, nullptr, 1080, 1920, 0, nullptr, nullptr,
+                                       nullptr, nullptr) == EPCQ_ERR_WORKSPACE,
+          "noalloc limit");
     check(epcq_scatter(nullptr, nullptr, nullptr, 10, 4, 4, 0, nullptr, nullptr, nullptr) == EPCQ_ERR_INVALID, "bins");
     check(epcq_knn_sampling(nullptr, 0, &cam, &o, nullptr, nullptr, nullptr, nullptr, 0, 7, 2, nullptr, nullptr,
                             nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, 0,
