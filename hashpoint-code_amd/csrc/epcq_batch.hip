@@ -45,15 +45,26 @@ namespace {
 
 __global__ void batch_reset_kernel(BatchHeader* h) { h->counter = 0; }
 
-__global__ __launch_bounds__(kBuildBlock) void batch_count_kernel(const BatchHeader* __restrict__ h,
-                                                                  const BatchView* __restrict__ views,
-                                                                  const int* __restrict__ prefix) {
-    __shared__ int s_hist[kMaxBins];
-    const int n_items = h->items[kCount], V = h->V;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const int v = find_view(prefix, V, item);
-        count_body(views[v].b, item - prefix[v], s_hist);
+// Calls fn(view, chunk) for this block's share of the (view, chunk) items of
+// its XCD: the views the XCD was given (xlist) and their chunk prefix
+// (xprefix); the launch's grid is a multiple of kXcds.
+template <class Fn>
+__device__ __forceinline__ void for_each_xcd_chunk(const int* __restrict__ tables, int M, Fn&& fn) {
+    const int x = (int)blockIdx.x % kXcds, loc = (int)blockIdx.x / kXcds, nloc = (int)gridDim.x / kXcds;
+    const int* xprefix = tables + (kStages + x) * M;
+    const int* xlist = tables + (kStages + kXcds + x) * M;
+    const int nv = xlist[M - 1];
+    const int n_items = xprefix[nv];
+    for (int item = loc; item < n_items; item += nloc) {
+        const int j = find_view(xprefix, nv, item);
+        fn(xlist[j], item - xprefix[j]);
     }
+}
+
+__global__ __launch_bounds__(kBuildBlock) void batch_count_kernel(const BatchView* __restrict__ views,
+                                                                  const int* __restrict__ tables, int M) {
+    __shared__ int s_hist[kMaxBins];
+    for_each_xcd_chunk(tables, M, [&](int v, int chunk) { count_body(views[v].b, chunk, s_hist); });
 }
 
 __global__ __launch_bounds__(kBuildBlock) void batch_colpart_kernel(const BatchHeader* __restrict__ h,
@@ -88,15 +99,10 @@ __global__ __launch_bounds__(kBuildBlock) void batch_binscan_kernel(const BatchH
     for (int v = blockIdx.x; v < V; v += gridDim.x) bin_scan_body(views[v].b, s_wave, &s_carry);
 }
 
-__global__ __launch_bounds__(kBuildBlock) void batch_bin_kernel(const BatchHeader* __restrict__ h,
-                                                                const BatchView* __restrict__ views,
-                                                                const int* __restrict__ prefix) {
+__global__ __launch_bounds__(kBuildBlock) void batch_bin_kernel(const BatchView* __restrict__ views,
+                                                                const int* __restrict__ tables, int M) {
     __shared__ int s_cur[kMaxBins];
-    const int n_items = h->items[kCount], V = h->V;
-    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-        const int v = find_view(prefix, V, item);
-        bin_body(views[v].b, item - prefix[v], s_cur);
-    }
+    for_each_xcd_chunk(tables, M, [&](int v, int chunk) { bin_body(views[v].b, chunk, s_cur); });
 }
 
 // Cone of pixel p (as bucket_cone_body) computed by a group of G = 2^gs
@@ -284,7 +290,7 @@ namespace {
 size_t header_bytes(int max_views) {
     size_t off = align_up(sizeof(BatchHeader), 256);
     off = align_up(off + (size_t)max_views * sizeof(BatchView), 256);
-    off = align_up(off + (size_t)kStages * (max_views + 1) * sizeof(int), 256);
+    off = align_up(off + (size_t)kTableRows * (max_views + 1) * sizeof(int), 256);
     return off;
 }
 
@@ -360,12 +366,12 @@ int enqueue_all(epcq_batch* b, hipStream_t st, hipEvent_t* ev = nullptr) {
     const dim3 blk(kBuildBlock), gb(b->grid_build);
     if (ev) (void)hipEventRecord(ev[0], st);
     batch_reset_kernel<<<1, 1, 0, st>>>(h);
-    batch_count_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
+    batch_count_kernel<<<gb, blk, 0, st>>>(dv, pf, M);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 0);
     batch_bintotal_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kTot * M);
     batch_binscan_kernel<<<gb, blk, 0, st>>>(h, dv);
     batch_colpart_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCol * M, 1);
-    batch_bin_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kCount * M);
+    batch_bin_kernel<<<gb, blk, 0, st>>>(dv, pf, M);
     if (ev) (void)hipEventRecord(ev[1], st);
     batch_fine_kernel<<<gb, blk, 0, st>>>(h, dv, pf + kFine * M, b->radius, b->opt.radius_threshold);
     if (ev) (void)hipEventRecord(ev[2], st);
@@ -559,8 +565,25 @@ extern "C" int epcq_batch_set_views(epcq_batch* b, const epcq_view_desc* views, 
         bv.pix_off = ray_off[i];
     }
     const int M = b->max_views + 1;
-    b->prefix.assign((size_t)kStages * M, 0);
+    b->prefix.assign((size_t)kTableRows * M, 0);
     int* P = b->prefix.data();
+    // each view's chunks go to one XCD: largest first to the least loaded
+    {
+        std::vector<int> byload(V);
+        std::iota(byload.begin(), byload.end(), 0);
+        std::stable_sort(byload.begin(), byload.end(),
+                         [&](int x, int y) { return b->views[x].b.nblk > b->views[y].b.nblk; });
+        int64_t load[kXcds] = {};
+        for (int r : byload) {
+            const int x = (int)(std::min_element(load, load + kXcds) - load);
+            int* xprefix = P + (kStages + x) * M;
+            int* xlist = P + (kStages + kXcds + x) * M;
+            const int j = xlist[M - 1]++;
+            xlist[j] = r;
+            xprefix[j + 1] = xprefix[j] + b->views[r].b.nblk;
+            load[x] += b->views[r].b.nblk;
+        }
+    }
     for (int r = 0; r < V; ++r) {
         const BatchView& bv = b->views[r];
         P[kCount * M + r + 1] = P[kCount * M + r] + bv.b.nblk;

@@ -14,6 +14,15 @@ constexpr int kMaxBins = 1 << kMaxBinBits;      // LDS cursors of count / bin
 constexpr int kMaxFineTile = 1 << 12;          // pixels per bin (fine LDS)
 constexpr int kStages = 6;                     // item spaces: count, col, scan, fine, pixels, tiles
 enum { kCount = 0, kCol = 1, kScan = 2, kFine = 3, kTot = 4, kTile = 5 };
+// The GPU's XCDs (8 on MI355X), each with its own L2; workgroup b of a launch
+// runs on XCD b % kXcds.  The projection and binning passes give every view
+// to one XCD, so the partial cache lines its binned records are appended to
+// fill up in that XCD's L2 instead of being written back from several.
+constexpr int kXcds = 8;
+// Rows of the item tables: kStages prefix rows over the views, then per XCD
+// a prefix row over its views' chunks and a row listing those views (their
+// count in the row's last entry).
+constexpr int kTableRows = kStages + 2 * kXcds;
 
 struct BatchView {
     BuildView b;
@@ -33,6 +42,10 @@ struct BatchHeader {
     int counter;    // next sample item
     SampleOut out;  // compact outputs of the current views (moves with the view set)
 };
+
+// Views whose item prefix the persistent sampler keeps in LDS (more views
+// search it in global memory).
+constexpr int kLdsViews = 256;
 
 // Largest v with prefix[v] <= item (prefix has V + 1 entries, prefix[0] = 0).
 __device__ __forceinline__ int find_view(const int* __restrict__ prefix, int V, int item) {
@@ -56,25 +69,31 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
     __shared__ int s_total;
     __shared__ int s_item;
     __shared__ int s_heavy[kMaxTileRays + 1];
+    __shared__ int s_prefix[kLdsViews + 1];
     const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
     const int n_items = h->items[kTile], V = h->V;
     const SampleOut out = h->out;
+    // the view prefix in LDS (its binary search runs once per tile)
+    const bool lds_prefix = V <= kLdsViews;
+    if (lds_prefix)
+        for (int i = threadIdx.x; i <= V; i += blockDim.x) s_prefix[i] = prefix[i];
+    const int* pf = lds_prefix ? s_prefix : prefix;
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(&h->counter, 1);
         __syncthreads();
         const int item = s_item;
         __syncthreads();
         if (item >= n_items) break;
-        const int v = find_view(prefix, V, item);
+        const int v = find_view(pf, V, item);
         const BatchView& bv = views[v];
         // a tile whose halo region exceeds the LDS reads global memory right
         // away: the densest views come first, so these long tiles overlap
         // with the rest of the batch
         const bool staged = sample_tile_body<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp,
-                                                                                item - prefix[v], bv.tiles_x,
+                                                                                item - pf[v], bv.tiles_x,
                                                                                 bv.ray_off, sm);
         if (!staged)
-            sample_tile_global<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp, item - prefix[v],
+            sample_tile_global<MODE, KMAX, SMAX, kLayCompact, PR>(o, out, bv.view, tp, item - pf[v],
                                                                   bv.tiles_x, bv.ray_off, sm);
     }
 }

@@ -379,6 +379,33 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
                          "quick_sampling");
 }
 
+extern "C" int epcq_quick_sampling_rays(const float* xyz, int64_t n, const epcq_camera* cam,
+                                        const epcq_options* opt, const int32_t* sorted_idx, const int32_t* hash,
+                                        int32_t mode, const int32_t* rays, int64_t R, float* sp_xyz, int32_t* pc_idx,
+                                        float* sp_t, uint8_t* mask, void* workspace, size_t workspace_bytes,
+                                        void* stream) {
+    if (int e = check_common(cam, opt, "quick_sampling")) return e;
+    if (mode != EPCQ_MODE_COMPAT && mode != EPCQ_MODE_KNN)
+        return fail(EPCQ_ERR_INVALID, "quick_sampling: unknown mode");
+    if (opt->num_sample_points_per_ray < 0 || opt->num_point_cloud_per_sp < 0)
+        return fail(EPCQ_ERR_INVALID, "quick_sampling: negative output size");
+    if (mode == EPCQ_MODE_KNN && opt->num_point_cloud_per_sp > 32)
+        return fail(EPCQ_ERR_INVALID, "quick_sampling: num_point_cloud_per_sp > 32 in KNN mode");
+    if (R < 0 || (R > 0 && !rays)) return fail(EPCQ_ERR_INVALID, "quick_sampling: bad ray list");
+    if (opt->num_sample_points_per_ray == 0 || R == 0) return EPCQ_OK;
+    const int bound = epcq_max_live_samples(opt, opt->radius_threshold);
+    SampleOpts o = make_opts(xyz, opt, nullptr);
+    SampleOut out{};
+    out.sp_xyz = sp_xyz;
+    out.pc_idx = pc_idx;
+    out.sp_t = sp_t;
+    out.mask = mask;
+    hipStream_t st = as_stream(stream);
+    ViewArgs view = make_view(cam, sorted_idx, hash);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    return launch_sample(o, out, view, rays, R, mode, bound, st, "quick_sampling");
+}
+
 extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera* cam, const epcq_options* opt,
                                  const int32_t* sorted_idx, const int32_t* hash, const float* radius,
                                  const int32_t* rays, int64_t R, int32_t mode, int32_t s_max, int32_t* n_samples,

@@ -97,6 +97,8 @@ ABI_SYMBOLS = {
     "epcq_build_hashtable_noalloc": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P]),
     "epcq_quick_sampling": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
                                    _I32, _P, _P, _P, _P, _P, _SZ, _P]),
+    "epcq_quick_sampling_rays": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P,
+                                        _I32, _P, _I64, _P, _P, _P, _P, _P, _SZ, _P]),
     "epcq_sampling_workspace_size": (_SZ, [_I64, _I32, _I32]),
     "epcq_quick_query": (_I32, [_P, _I64, ctypes.POINTER(_Camera), ctypes.POINTER(_Options), _P, _P, _P,
                                 _I64, _P, _P]),
@@ -106,6 +108,9 @@ ABI_SYMBOLS = {
     "epcq_max_live_samples": (_I32, [ctypes.POINTER(_Options), _F32]),
     "epcq_idw_features": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
     "epcq_debug_expf": (_I32, [_P, _P, _I64, _P]),
+    "epcq_clamp_t": (_I32, [_P, _I64, _I32, _F32, _P, _P]),
+    "epcq_render_rays": (_I32, [_P, _P, _F32, _P, _P, _I64, _I32, _I32, _P, _P]),
+    "epcq_render_rays_backward": (_I32, [_P, _P, _F32, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P]),
     "epcq_batch_arena_bytes": (_SZ, [ctypes.POINTER(_ViewDesc), _I32, _I32, ctypes.POINTER(_Options), _I32, _I32]),
     "epcq_batch_create": (_I32, [ctypes.POINTER(_Options), _I32, _I32, _I32, _P, _F32, ctypes.POINTER(_Tuning), _P,
                                  _SZ, ctypes.POINTER(ctypes.c_void_p)]),
@@ -397,30 +402,48 @@ def _nbytes(t):
     return 0 if t is None else t.numel() * t.element_size()
 
 
-def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, mode="compat", cull=True):
+def quick_sampling(neural_points, cam, options, sorted_point_idx_list, hashtable, mode="compat", cull=True,
+                   rays=None):
     """quick_sampling (epcq/csrc/epcq_render_topk.cu:341-417).
 
     Returns (output_sp_xyz float32 [H,W,S,3], output_pc_idx int32 [H,W,S,Kc],
     output_sp_t float32 [H,W,S], mask_out bool [H,W,S]).  `mode` "compat"
     keeps the reference's neighbour buffer; "knn" returns the Kc nearest by
-    (distance, index)."""
+    (distance, index).  With `rays` (int32 pixel ids v * W + u, [R]) only
+    those rays are sampled and the outputs are [R,S,3], [R,S,Kc], [R,S],
+    [R,S]: row r equals the full image's row at rays[r] (an id outside the
+    image gives an empty row)."""
     device = _check_spec_inputs(neural_points, cam, sorted_point_idx_list, hashtable)
     H, W = int(cam.height), int(cam.width)
     hashtable = _hashtable_view(hashtable, H, W)
     S = int(options.num_sample_points_per_ray)
     Kc = int(options.num_point_cloud_per_sp)
     xyz = neural_points.xyz_data
-    sp_xyz = torch.empty((H, W, S, 3), dtype=torch.float32, device=device)
-    sp_t = torch.empty((H, W, S), dtype=torch.float32, device=device)
-    pc_idx = torch.empty((H, W, S, Kc), dtype=torch.int32, device=device)
-    mask = torch.empty((H, W, S), dtype=torch.bool, device=device)
+    if rays is not None:
+        _check_input(rays, "rays")
+        _check_dtype(rays, torch.int32, "rays")
+        _check_same_device(device, ("rays", rays))
+        lead = (rays.numel(),)
+    else:
+        lead = (H, W)
+    sp_xyz = torch.empty(lead + (S, 3), dtype=torch.float32, device=device)
+    sp_t = torch.empty(lead + (S,), dtype=torch.float32, device=device)
+    pc_idx = torch.empty(lead + (S, Kc), dtype=torch.int32, device=device)
+    mask = torch.empty(lead + (S,), dtype=torch.bool, device=device)
     ws = _workspace(xyz.size(0), H, W, device, cull)
     camera, opts = _camera_struct(cam), _options_struct(options)
     with torch.cuda.device(device):
-        _raise_on(_lib.epcq_quick_sampling(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
-                                           _ptr(sorted_point_idx_list), _ptr(hashtable), _MODES[mode],
-                                           _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _ptr(ws),
-                                           _nbytes(ws), _stream(device)))
+        if rays is None:
+            _raise_on(_lib.epcq_quick_sampling(_ptr(xyz), xyz.size(0), ctypes.byref(camera), ctypes.byref(opts),
+                                               _ptr(sorted_point_idx_list), _ptr(hashtable), _MODES[mode],
+                                               _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _ptr(ws),
+                                               _nbytes(ws), _stream(device)))
+        else:
+            _raise_on(_lib.epcq_quick_sampling_rays(_ptr(xyz), xyz.size(0), ctypes.byref(camera),
+                                                    ctypes.byref(opts), _ptr(sorted_point_idx_list),
+                                                    _ptr(hashtable), _MODES[mode], _ptr(rays), rays.numel(),
+                                                    _ptr(sp_xyz), _ptr(pc_idx), _ptr(sp_t), _ptr(mask), _ptr(ws),
+                                                    _nbytes(ws), _stream(device)))
     return sp_xyz, pc_idx, sp_t, mask
 
 
@@ -585,6 +608,78 @@ def idw_features(xyz, feat, sp_xyz, nb_idx):
         _raise_on(_lib.epcq_idw_features(_ptr(xyz), _ptr(feat), int(feat.dtype == torch.bfloat16), F,
                                          _ptr(sp_xyz), _ptr(nb_idx), M, K, _ptr(w), _ptr(agg), _stream(dev)))
     return w, agg
+
+
+def _check_rows(name, x, R, S, last=None):
+    _check_input(x, name)
+    _check_dtype(x, torch.float32, name)
+    shape = (R, S) if last is None else (R, S, last)
+    if tuple(x.shape) != shape:
+        raise RuntimeError(f"{name} must have shape {list(shape)}, found {list(x.shape)}")
+
+
+def clamp_t(t_val, vsize=0.004):
+    """clamp_t (epcq/epcq.py:508-515): per-sample intervals [R,S] of the
+    sample depths t_val [R,S] ([R,0] when S = 1, as the reference's slicing
+    gives)."""
+    _check_input(t_val, "t_val")
+    _check_dtype(t_val, torch.float32, "t_val")
+    if t_val.dim() != 2:
+        raise RuntimeError("t_val must have shape [R, S]")
+    R, S = t_val.shape
+    out = torch.empty((R, 0 if S == 1 else S), dtype=torch.float32, device=t_val.device)
+    with torch.cuda.device(t_val.device):
+        _raise_on(_lib.epcq_clamp_t(_ptr(t_val), R, S, float(vsize), _ptr(out), _stream(t_val.device)))
+    return out
+
+
+def render_rays(sigmas, rgbs, white_bg=False, deltas=None, t_val=None, vsize=0.004):
+    """render (epcq/epcq.py:517-530) of sigmas [R,S] and rgbs [R,S,3] with
+    intervals `deltas` [R,S], or clamp_t(t_val, vsize) applied on the fly
+    when `t_val` is given instead.  Returns ray colours [R,3] (no autograd;
+    see epcq.epcq.render for the differentiable form)."""
+    if (deltas is None) == (t_val is None):
+        raise RuntimeError("render_rays: give exactly one of deltas and t_val")
+    _check_input(sigmas, "sigmas")
+    if sigmas.dim() != 2:
+        raise RuntimeError("sigmas must have shape [R, S]")
+    R, S = sigmas.shape
+    _check_rows("sigmas", sigmas, R, S)
+    _check_rows("rgbs", rgbs, R, S, 3)
+    src = deltas if deltas is not None else t_val
+    _check_rows("deltas" if deltas is not None else "t_val", src, R, S)
+    dev = sigmas.device
+    _check_same_device(dev, ("rgbs", rgbs), ("deltas/t_val", src))
+    out = torch.empty((R, 3), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_render_rays(_ptr(t_val), _ptr(deltas), float(vsize), _ptr(sigmas), _ptr(rgbs), R, S,
+                                        int(bool(white_bg)), _ptr(out), _stream(dev)))
+    return out
+
+
+def render_rays_backward(sigmas, rgbs, grad_out, white_bg=False, deltas=None, t_val=None, vsize=0.004):
+    """Gradients (d sigmas [R,S], d rgbs [R,S,3]) of render_rays for the
+    output gradient grad_out [R,3]."""
+    if (deltas is None) == (t_val is None):
+        raise RuntimeError("render_rays_backward: give exactly one of deltas and t_val")
+    R, S = sigmas.shape
+    _check_rows("sigmas", sigmas, R, S)
+    _check_rows("rgbs", rgbs, R, S, 3)
+    src = deltas if deltas is not None else t_val
+    _check_rows("deltas" if deltas is not None else "t_val", src, R, S)
+    _check_input(grad_out, "grad_out")
+    _check_dtype(grad_out, torch.float32, "grad_out")
+    if tuple(grad_out.shape) != (R, 3):
+        raise RuntimeError("grad_out must have shape [R, 3]")
+    dev = sigmas.device
+    _check_same_device(dev, ("rgbs", rgbs), ("deltas/t_val", src), ("grad_out", grad_out))
+    gs = torch.empty((R, S), dtype=torch.float32, device=dev)
+    gc = torch.empty((R, S, 3), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_render_rays_backward(_ptr(t_val), _ptr(deltas), float(vsize), _ptr(sigmas), _ptr(rgbs),
+                                                 _ptr(grad_out), R, S, int(bool(white_bg)), _ptr(gs), _ptr(gc),
+                                                 _stream(dev)))
+    return gs, gc
 
 
 def debug_expf(x):

@@ -7,9 +7,11 @@ NeuralPoints binds the ops of `epcq.csrc` exactly as the reference does
 (`scatter`, `scatter_hashtable`, `quick_sampling`); `pts2plane` runs the fused
 gfx950 build (projection, pixel keys, stable sort and hash table in one
 pipeline) instead of torch matmuls + torch.sort + scatter_hashtable, with
-pixel lists in the deterministic (pixel, index) order.  The rendering head of
-the reference module (positional encoding, the Aggregator MLP, clamp_t,
-render) comes after this path and is not part of this package.
+pixel lists in the deterministic (pixel, index) order.  The volume-rendering
+epilogue of the reference module (clamp_t :508-515, render :517-530) runs on
+gfx950 kernels too, with a backward pass for training; the Aggregator MLP and
+positional encoding between sampling and rendering are not part of this
+package.
 """
 from dataclasses import dataclass
 from typing import Optional, Union
@@ -194,11 +196,23 @@ class NeuralPoints(nn.Module):
                                     sorted_depth_list: torch.Tensor,
                                     hashtable: torch.Tensor,
                                     save_variables_for_debug: bool = False,
-                                    mode: str = "compat", cull: bool = True):
+                                    mode: str = "compat", cull: bool = True,
+                                    training_ray_index: Optional[torch.Tensor] = None):
         """(output_sp_xyz, output_pc_idx, output_sp_t, mask_out) of every
-        pixel; `save_variables_for_debug` is accepted and ignored."""
+        pixel; `save_variables_for_debug` is accepted and ignored.
+
+        With a non-empty `training_ray_index` ([R, >=2] integer (u, v, ...)
+        rows, as the reference's data loader builds them) only those rays are
+        sampled, and the outputs are [R,S,3], [R,S,Kc], [R,S], [R,S]: what the
+        reference's renderer obtains by indexing the full-image outputs with
+        [v, u] (opt/render_image.py:82-88), without sampling the rest."""
+        rays = None
+        if training_ray_index is not None and len(training_ray_index) != 0:
+            tri = training_ray_index.to(device=hashtable.device)
+            rays = (tri[:, 1].to(torch.int64) * int(cam.width) + tri[:, 0].to(torch.int64)).to(torch.int32)
+            rays = rays.contiguous()
         return self.quick_sampling(self._to_cpp(), cam._to_cpp(), opt._to_cpp(), sorted_point_idx_list,
-                                   hashtable, mode=mode, cull=cull)
+                                   hashtable, mode=mode, cull=cull, rays=rays)
 
     def knn_sampling(self, cam: CameraSpec, opt: RenderOptions, sorted_point_idx_list: torch.Tensor,
                      hashtable: torch.Tensor, rays: Optional[torch.Tensor] = None,
@@ -258,3 +272,43 @@ def calculate_weight(dist: torch.Tensor, mask: torch.Tensor = None):
     if mask is not None:
         inv = inv * mask.float()
     return inv / (inv.sum(dim=-1, keepdim=True) + 1e-6)
+
+
+def clamp_t(t_val: torch.Tensor, vsize: float = 0.004):
+    """Sample intervals of sample depths t_val [R,S] (epcq.py:508-515): the
+    difference to the next sample, vsize for the last, and vsize wherever the
+    difference is below 1e-8 or above 2 * vsize."""
+    return _C.clamp_t(t_val.contiguous(), vsize)
+
+
+class _Render(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, deltas, sigmas, rgbs, white_bg):
+        deltas = deltas.contiguous()
+        sigmas = sigmas.contiguous()
+        rgbs = rgbs.contiguous()
+        out = _C.render_rays(sigmas, rgbs, white_bg=white_bg, deltas=deltas)
+        ctx.save_for_backward(deltas, sigmas, rgbs)
+        ctx.white_bg = white_bg
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        deltas, sigmas, rgbs = ctx.saved_tensors
+        gs, gc = _C.render_rays_backward(sigmas, rgbs, grad_out.contiguous(), white_bg=ctx.white_bg, deltas=deltas)
+        return None, gs, gc, None
+
+
+def render(deltas: torch.Tensor, sigmas: torch.Tensor, rgbs: torch.Tensor, white_bg: bool = False):
+    """Ray colours [R,3] by alpha compositing (epcq.py:517-530): opacity
+    1 - exp(-sigma * delta), transmittance the exclusive running product of
+    (1 - opacity + 1e-10), background weighted by the transmittance before
+    the last sample.  Differentiable in sigmas [R,S] and rgbs [R,S,3] (the
+    intervals deltas [R,S] come from the samples and take no gradient).
+    Intervals of one sample per ray are empty ([R,0], see clamp_t): no
+    sample contributes and the colour is the background, as in the
+    reference."""
+    if deltas.dim() == 2 and deltas.shape[1] == 0:
+        base = sigmas.new_ones((sigmas.shape[0], 3)) if white_bg else sigmas.new_zeros((sigmas.shape[0], 3))
+        return base + 0.0 * (sigmas.sum() + rgbs.sum())  # keeps the graph: zero gradients
+    return _Render.apply(deltas, sigmas, rgbs, bool(white_bg))

@@ -163,6 +163,20 @@ int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camera* cam,
                         int32_t mode, float* sp_xyz, int32_t* pc_idx, float* sp_t, uint8_t* mask,
                         void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * quick_sampling of a ray list -- the training form of the reference's
+ * renderer, which samples the whole image and then keeps the rays listed in
+ * training_ray_index (opt/render_image.py:75-88).  rays [R] pixel ids
+ * (v * W + u); writes sp_xyz [R,S,3], pc_idx [R,S,Kc], sp_t [R,S] and
+ * mask [R,S], row r equal to row rays[r] of epcq_quick_sampling's outputs.
+ * Ids outside [0, H*W) give an empty row (zeros / -1 / false).
+ */
+int epcq_quick_sampling_rays(const float* xyz, int64_t n, const epcq_camera* cam,
+                             const epcq_options* opt, const int32_t* sorted_idx, const int32_t* hash,
+                             int32_t mode, const int32_t* rays, int64_t R, float* sp_xyz, int32_t* pc_idx,
+                             float* sp_t, uint8_t* mask, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
 /* Workspace of the samplers for n points and an H x W image: per-pixel
  * bucket cones that let the walks skip neighbour pixels provably outside the
  * radius, and a pixel-ordered copy of the points.  Optional: with a NULL or
@@ -215,6 +229,25 @@ int32_t epcq_max_live_samples(const epcq_options* opt, float r_max);
 int epcq_idw_features(const float* xyz, const void* feat, int32_t feat_bf16, int32_t F,
                       const float* sp_xyz, const int32_t* nb_idx, int64_t M, int32_t K,
                       float* w, float* agg, void* stream);
+
+/*
+ * Volume-rendering epilogue: clamp_t and render (epcq/epcq.py:508-530, also
+ * utils/utils.py:422-467) as the reference renderer applies them to the
+ * samples (opt/render_image.py:101-109), plus render's backward.
+ *   clamp_t: delta [R,S] from sample depths t [R,S] (neighbour differences,
+ *            vsize for the last and where below 1e-8 or above 2 vsize).
+ *   render_rays: out_rgb [R,3] from sigma [R,S] and rgb [R,S,3]; give either
+ *            t (clamp_t is then applied on the fly) or delta, not both.
+ *            white_bg != 0 adds the remaining transmittance times white.
+ *   render_rays_backward: grad_sigma [R,S] and grad_rgb [R,S,3] from
+ *            grad_rgb_out [R,3] (same t / delta convention).
+ */
+int epcq_clamp_t(const float* t, int64_t R, int32_t S, float vsize, float* delta, void* stream);
+int epcq_render_rays(const float* t, const float* delta, float vsize, const float* sigma, const float* rgb,
+                     int64_t R, int32_t S, int32_t white_bg, float* out_rgb, void* stream);
+int epcq_render_rays_backward(const float* t, const float* delta, float vsize, const float* sigma,
+                              const float* rgb, const float* grad_rgb_out, int64_t R, int32_t S,
+                              int32_t white_bg, float* grad_sigma, float* grad_rgb, void* stream);
 
 /*
  * Batched multi-view sampling (no counterpart in the reference, which renders

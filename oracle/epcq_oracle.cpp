@@ -623,4 +623,98 @@ int orc_idw(const float* xyz, const void* feat, int32_t feat_bf16, int32_t F, co
     return 0;
 }
 
+
+/* ---- volume-rendering epilogue: clamp_t and render (epcq/epcq.py:508-530) ----
+ * One ray at a time, samples front to back, every operation of the
+ * reference's formulas in its order, one rounding each:
+ *   clamp_t  t_dist = t[k+1] - t[k] (vsize appended for the last), mask =
+ *            (t_dist < 1e-8) | (t_dist > 2 vsize), t_dist (1 - mask) + mask vsize
+ *   render   opacity = 1 - exp(-sigma delta); acc = cumprod(1 - opacity +
+ *            1e-10) shifted by one sample (acc_0 = 1); colour = sum of
+ *            opacity acc rgb, plus acc_{S-1} bg.
+ * exp is orc_expf.  The backward is the derivative of these formulas with
+ * respect to sigma and rgb, in the order the device kernel evaluates it:
+ *   d rgb_k   = opacity_k acc_k g
+ *   d sigma_k = delta_k e_k (acc_k (g . rgb_k) - (R_k + [k < S-1] acc_{S-1} (g . bg)) / a_k)
+ * with e_k = exp(-sigma_k delta_k), a_k = 1 - opacity_k + 1e-10, and R_k =
+ * total - (running prefix up to k) of opacity_i acc_i (g . rgb_i). */
+static float orc_delta(const float* t, const float* delta, int32_t S, int32_t k, float vsize) {
+    if (delta) return delta[k];
+    const float d = k + 1 < S ? t[k + 1] - t[k] : vsize;
+    const float m = (d < 1e-8f || d > 2.0f * vsize) ? 1.0f : 0.0f;
+    return d * (1.0f - m) + m * vsize;
+}
+
+int orc_clamp_t(const float* t, int64_t R, int32_t S, float vsize, float* delta) {
+    if (S == 1) return 0;  // the reference's t_dist[:, :1] of an empty slice: no interval
+    for (int64_t r = 0; r < R; ++r)
+        for (int32_t k = 0; k < S; ++k) delta[r * S + k] = orc_delta(t + r * S, nullptr, S, k, vsize);
+    return 0;
+}
+
+int orc_render(const float* t, const float* delta, float vsize, const float* sigma, const float* rgb, int64_t R,
+               int32_t S, int32_t white_bg, float* out) {
+    const float bg = white_bg ? 1.0f : 0.0f;
+    for (int64_t r = 0; r < R; ++r) {
+        const float* tr = t ? t + r * S : nullptr;
+        const float* dr = delta ? delta + r * S : nullptr;
+        float acc = 1.0f, acc_last = 1.0f, c[3] = {0.0f, 0.0f, 0.0f};
+        const int32_t live = (t && S == 1) ? 0 : S;  // S = 1 from t: no interval, no contribution
+        for (int32_t k = 0; k < live; ++k) {
+            const float dk = orc_delta(tr, dr, S, k, vsize);
+            const float op = 1.0f - orc_expf(-sigma[r * S + k] * dk);
+            const float w = op * acc;
+            for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + w * rgb[(r * S + k) * 3 + ch];
+            acc_last = acc;
+            acc = acc * ((1.0f - op) + 1e-10f);
+        }
+        for (int ch = 0; ch < 3; ++ch) out[3 * r + ch] = c[ch] + acc_last * bg;
+    }
+    return 0;
+}
+
+int orc_render_backward(const float* t, const float* delta, float vsize, const float* sigma, const float* rgb,
+                        const float* grad_out, int64_t R, int32_t S, int32_t white_bg, float* grad_sigma,
+                        float* grad_rgb) {
+    const float bg = white_bg ? 1.0f : 0.0f;
+    std::vector<float> dk(S > 0 ? S : 1), op(S > 0 ? S : 1), acc(S > 0 ? S : 1), gc(S > 0 ? S : 1);
+    for (int64_t r = 0; r < R; ++r) {
+        const float* tr = t ? t + r * S : nullptr;
+        const float* dr = delta ? delta + r * S : nullptr;
+        const float* g = grad_out + 3 * r;
+        if (t && S == 1) {
+            grad_sigma[r] = 0.0f;
+            for (int ch = 0; ch < 3; ++ch) grad_rgb[3 * r + ch] = 0.0f;
+            continue;
+        }
+        float a = 1.0f, total = 0.0f;
+        for (int32_t k = 0; k < S; ++k) {
+            dk[k] = orc_delta(tr, dr, S, k, vsize);
+            op[k] = 1.0f - orc_expf(-sigma[r * S + k] * dk[k]);
+            const float* ck = rgb + (r * S + k) * 3;
+            float dot = g[0] * ck[0];
+            dot = dot + g[1] * ck[1];
+            dot = dot + g[2] * ck[2];
+            gc[k] = dot;
+            acc[k] = a;
+            total = total + op[k] * a * gc[k];
+            a = a * ((1.0f - op[k]) + 1e-10f);
+        }
+        const float acc_last = S > 0 ? acc[S - 1] : 1.0f;
+        const float gbg = ((g[0] + g[1]) + g[2]) * bg;
+        float prefix = 0.0f;
+        for (int32_t k = 0; k < S; ++k) {
+            const float e = orc_expf(-sigma[r * S + k] * dk[k]);
+            const float opk = 1.0f - e;
+            const float ak = (1.0f - opk) + 1e-10f;
+            const float wk = opk * acc[k];
+            prefix = prefix + wk * gc[k];
+            const float after = (total - prefix) + (k + 1 < S ? acc_last * gbg : 0.0f);
+            grad_sigma[r * S + k] = dk[k] * e * (acc[k] * gc[k] - after / ak);
+            for (int ch = 0; ch < 3; ++ch) grad_rgb[(r * S + k) * 3 + ch] = wk * g[ch];
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"

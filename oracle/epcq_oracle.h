@@ -127,6 +127,19 @@ int orc_idw(const float* xyz, const void* feat, int32_t feat_bf16, int32_t F,
             const float* sp_xyz, const int32_t* nb_idx, int64_t M, int32_t K,
             float* w, float* agg);
 
+/*
+ * Volume-rendering epilogue (epcq/epcq.py:508-530): clamp_t delta [R,S] from
+ * t [R,S]; render out [R,3] from sigma [R,S], rgb [R,S,3] and either t
+ * (clamped on the fly) or delta (the other NULL); its backward grad_sigma
+ * [R,S], grad_rgb [R,S,3] from grad_out [R,3].
+ */
+int orc_clamp_t(const float* t, int64_t R, int32_t S, float vsize, float* delta);
+int orc_render(const float* t, const float* delta, float vsize, const float* sigma, const float* rgb, int64_t R,
+               int32_t S, int32_t white_bg, float* out);
+int orc_render_backward(const float* t, const float* delta, float vsize, const float* sigma, const float* rgb,
+                        const float* grad_out, int64_t R, int32_t S, int32_t white_bg, float* grad_sigma,
+                        float* grad_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -223,3 +223,34 @@ def idw(xyz, feat, sp_xyz, nb_idx, bf16=False):
     lib().orc_idw(_p(xyz), _p(feat), int(bf16), int(F), _p(sp_xyz), _p(nb_idx), ctypes.c_int64(M), int(K), _p(w),
                   _p(agg))
     return w, agg
+
+
+def clamp_t(t, vsize=0.004):
+    t = _c(t, np.float32)
+    R, S = t.shape
+    out = np.empty((R, 0 if S == 1 else S), np.float32)
+    lib().orc_clamp_t(_p(t), ctypes.c_int64(R), int(S), ctypes.c_float(vsize), _p(out))
+    return out
+
+
+def render(sigma, rgb, white_bg=False, t=None, delta=None, vsize=0.004):
+    sigma, rgb = _c(sigma, np.float32), _c(rgb, np.float32)
+    R, S = sigma.shape
+    t = _c(t, np.float32) if t is not None else None
+    delta = _c(delta, np.float32) if delta is not None else None
+    out = np.empty((R, 3), np.float32)
+    lib().orc_render(_p(t), _p(delta), ctypes.c_float(vsize), _p(sigma), _p(rgb), ctypes.c_int64(R), int(S),
+                     int(bool(white_bg)), _p(out))
+    return out
+
+
+def render_backward(sigma, rgb, grad_out, white_bg=False, t=None, delta=None, vsize=0.004):
+    sigma, rgb, grad_out = _c(sigma, np.float32), _c(rgb, np.float32), _c(grad_out, np.float32)
+    R, S = sigma.shape
+    t = _c(t, np.float32) if t is not None else None
+    delta = _c(delta, np.float32) if delta is not None else None
+    gs = np.empty((R, S), np.float32)
+    gc = np.empty((R, S, 3), np.float32)
+    lib().orc_render_backward(_p(t), _p(delta), ctypes.c_float(vsize), _p(sigma), _p(rgb), _p(grad_out),
+                              ctypes.c_int64(R), int(S), int(bool(white_bg)), _p(gs), _p(gc))
+    return gs, gc

@@ -1,0 +1,65 @@
+"""The volume-rendering epilogue on the GPU (clamp_t, render, and render's
+backward through torch autograd) against the oracle bit for bit (same
+operation order, same exponential) and against the reference's formulas in
+torch fp32 within 1e-5 relative (torch's exp, cumprod and sum round
+differently)."""
+import numpy as np
+import pytest
+import torch
+
+import epcq.csrc as C
+from epcq.epcq import clamp_t, render
+from oracle import oracle
+from scenes import bits
+from test_oracle_render import random_rays, torch_reference
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("R,S", [(1, 1), (1000, 16), (4097, 7), (300, 64)])
+@pytest.mark.parametrize("white", [False, True])
+def test_render_forward(dev, R, S, white):
+    t, sigma, rgb = random_rays(R, S, 10 + S)
+    td, sd, cd = (torch.from_numpy(a).to(dev) for a in (t, sigma, rgb))
+    print(f"render R={R} S={S} white={white}", flush=True)
+    delta = clamp_t(td)
+    assert np.array_equal(bits(delta.cpu().numpy()), bits(oracle.clamp_t(t)))
+    out = render(delta, sd, cd, white_bg=white)
+    fused = C.render_rays(sd, cd, white_bg=white, t_val=td)
+    torch.cuda.synchronize()
+    ref = oracle.render(sigma, rgb, white, t=t)
+    assert np.array_equal(bits(out.cpu().numpy()), bits(ref))
+    assert np.array_equal(bits(fused.cpu().numpy()), bits(ref))
+    tref = torch_reference(td, sd, cd, white)
+    np.testing.assert_allclose(out.cpu().numpy(), tref.cpu().numpy(), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("white", [False, True])
+def test_render_backward(dev, white):
+    R, S = 2000, 16
+    t, sigma, rgb = random_rays(R, S, 5)
+    g = np.random.default_rng(6).normal(size=(R, 3)).astype(np.float32)
+    td = torch.from_numpy(t).to(dev)
+    sd = torch.from_numpy(sigma).to(dev).requires_grad_(True)
+    cd = torch.from_numpy(rgb).to(dev).requires_grad_(True)
+    print(f"render backward white={white}", flush=True)
+    out = render(clamp_t(td), sd, cd, white_bg=white)
+    out.backward(torch.from_numpy(g).to(dev))
+    torch.cuda.synchronize()
+    gs, gc = oracle.render_backward(sigma, rgb, g, white, t=t)
+    assert np.array_equal(bits(sd.grad.cpu().numpy()), bits(gs))
+    assert np.array_equal(bits(cd.grad.cpu().numpy()), bits(gc))
+    # against autograd of the reference formulas (float64 on the CPU)
+    st = torch.from_numpy(sigma).double().requires_grad_(True)
+    ct = torch.from_numpy(rgb).double().requires_grad_(True)
+    torch_reference(torch.from_numpy(t).double(), st, ct, white).backward(torch.from_numpy(g).double())
+    np.testing.assert_allclose(cd.grad.cpu().numpy(), ct.grad.numpy(), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(sd.grad.cpu().numpy(), st.grad.numpy(), rtol=2e-4, atol=2e-6)
+
+
+def test_render_argument_errors(dev):
+    z = torch.zeros((4, 3), device=dev)
+    with pytest.raises(RuntimeError, match="exactly one"):
+        C.render_rays(z, torch.zeros((4, 3, 3), device=dev))
+    with pytest.raises(RuntimeError, match="rgbs must have shape"):
+        C.render_rays(z, torch.zeros((4, 3), device=dev), deltas=z)
