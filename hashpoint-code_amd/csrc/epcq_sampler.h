@@ -37,6 +37,7 @@
 #pragma once
 
 #include <climits>
+#include <type_traits>
 
 #include "epcq_device.h"
 
@@ -465,16 +466,77 @@ struct NbBuf {
     }
 };
 
+// The reference's 8-entry buffer (quick_search_nearby_pc, :50-73) for the
+// one-lane sampler when only the neighbour indices are output: the indices
+// go straight to the sample's output row as the buffer takes them (entry q
+// of the buffer is column q of the row), and only the distances stay in
+// registers, as a shift register while the buffer fills (r[n - 1 - q] holds
+// entry q, newest first), so no update depends on a per-lane index and
+// nothing branches on one.  Same semantics as NbBuf in compat mode: while
+// filling, the minimum is tracked; once full, a closer point replaces the
+// tracked entry, after which the maximum is tracked.
+template <int KMAX>
+struct CompatStream {
+    float r[KMAX];
+    int n;      // entries taken (<= KMAX)
+    int maxi;   // buffer position of the tracked entry
+    float maxd;
+    int* row;   // the sample's output row, kc columns
+    int kc;
+
+    __device__ __forceinline__ void init(int* out_row, int out_kc) {
+        n = 0;
+        maxi = 0;
+        maxd = 1e3f;
+        row = out_row;
+        kc = out_kc;
+#pragma unroll
+        for (int b = 0; b < KMAX; ++b) r[b] = 0.0f;
+    }
+
+    __device__ __forceinline__ void add(float d, int id) {
+        if (n < KMAX) {
+#pragma unroll
+            for (int b = KMAX - 1; b > 0; --b) r[b] = r[b - 1];
+            r[0] = d;
+            if (n < kc) row[n] = id;
+            if (maxd > d) {
+                maxd = d;
+                maxi = n;
+            }
+            ++n;
+        } else if (d < maxd) {
+            const int reg = KMAX - 1 - maxi;
+#pragma unroll
+            for (int b = 0; b < KMAX; ++b) r[b] = b == reg ? d : r[b];
+            if (maxi < kc) row[maxi] = id;
+            maxd = d;
+            // the maximum, first position on ties (registers in reverse)
+#pragma unroll
+            for (int q = 0; q < KMAX; ++q) {
+                const float x = r[KMAX - 1 - q];
+                const bool gt = x > maxd;
+                maxd = gt ? x : maxd;
+                maxi = gt ? q : maxi;
+            }
+        }
+    }
+
+    // -1 in the row's columns the buffer left empty.
+    __device__ __forceinline__ void finish() const {
+        for (int q = n; q < kc; ++q) row[q] = -1;
+    }
+};
+
 // One walk over the candidates of the window feeding the buffers of the
 // first `ns` (<= NS) sample points sp[], over the lanes set in `mask`
 // (buckets that may hold a point within reach of a sample point).  Compat
 // mode walks the k*k lanes with their border duplicates, like
 // quick_search_nearby_pc (:39-47); KNN mode walks the distinct window pixels
 // (the caller restricts `mask` to inside_lanes).
-template <int MODE, int KMAX, int NS, class Src>
+template <int MODE, int NS, class Src, class Buf>
 __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
-                                               const float (&sp)[NS][3], int ns, LaneMask mask,
-                                               NbBuf<MODE, KMAX> (&buf)[NS]) {
+                                               const float (&sp)[NS][3], int ns, LaneMask mask, Buf (&buf)[NS]) {
     auto visit = [&](const float4& p, int jp) {
         const int id = __float_as_int(p.w);
         const float lim = point_limit(o, src, jp, p);
@@ -838,9 +900,12 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
         // with a sample bound above 2 keep kRegSamples depths in registers
         // and re-read later ones from the outputs this thread wrote.
         const int nstore = min(SMAX < kRegSamples ? min(ns, SMAX) : ns, dense ? o.S : out.s_max);
+        // compat mode without features streams the indices to the outputs
+        constexpr bool stream = MODE == EPCQ_MODE_COMPAT && LAY != kLayFeat;
+        using Buf = typename std::conditional<stream, CompatStream<KMAX>, NbBuf<MODE, KMAX>>::type;
         for (int g = 0; g < nstore; g += NS) {
             float pend[NS][3];
-            NbBuf<MODE, KMAX> buf[NS];
+            Buf buf[NS];
             const int cnt = min(NS, nstore - g);
             float tabs = 0.0f;
 #pragma unroll
@@ -854,7 +919,15 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
                 pend[q][1] = ray.o[1] + tq * ray.d[1];
                 pend[q][2] = ray.o[2] + tq * ray.d[2];
                 if (q < cnt) tabs = fmaxf(tabs, fabsf(tq));
-                buf[q].init();
+                if constexpr (stream) {
+                    // a buffer beyond the stored samples (q >= cnt) never takes a point
+                    const int so = min(g + q, nstore - 1);
+                    int* row = dense ? out.pc_idx + (ri * o.S + so) * (int64_t)o.Kc
+                                     : out.nb_idx + (ri * out.s_max + so) * (int64_t)o.Kc;
+                    buf[q].init(row, o.Kc);
+                } else {
+                    buf[q].init();
+                }
             }
             // A sample point o + t d rounds to within ~eps (2 |t| |d|_1 + |o|_1)
             // of the ray and |sp - p| to a few ulp, so a bucket whose cone
@@ -864,10 +937,14 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
             if (o.cull && o.L <= 64)
                 nmask |= recheck_lanes(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1, emask & ~hmask);
             if (MODE != EPCQ_MODE_COMPAT && o.L <= 64) nmask &= inside_lanes(o, u, v, W, H);
-            neighbour_pass<MODE, KMAX, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
+            neighbour_pass<MODE, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
 #pragma unroll
-            for (int q = 0; q < NS; ++q)
-                if (q < cnt) write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
+            for (int q = 0; q < NS; ++q) {
+                if (q < cnt) {
+                    if constexpr (stream) buf[q].finish();
+                    else write_neighbours<MODE, KMAX, LAY>(o, out, ray, buf[q], ri, g + q);
+                }
+            }
         }
     }
 
