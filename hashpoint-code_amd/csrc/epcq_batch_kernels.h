@@ -11,7 +11,7 @@
 namespace epcq {
 
 constexpr int kMaxBins = 1 << kMaxBinBits;      // LDS cursors of count / bin
-constexpr int kMaxFineTile = 1 << 12;          // pixels per bin (fine LDS)
+constexpr int kMaxFineTile = 1 << 11;          // pixels per bin (fine LDS; 16 KB keeps 8 blocks per CU)
 constexpr int kStages = 6;                     // item spaces: count, col, scan, fine, pixels, tiles
 enum { kCount = 0, kCol = 1, kScan = 2, kFine = 3, kTot = 4, kTile = 5 };
 // The GPU's XCDs (8 on MI355X), each with its own L2; workgroup b of a launch
@@ -43,10 +43,6 @@ struct BatchHeader {
     SampleOut out;  // compact outputs of the current views (moves with the view set)
 };
 
-// Views whose item prefix the persistent sampler keeps in LDS (more views
-// search it in global memory).
-constexpr int kLdsViews = 256;
-
 // Largest v with prefix[v] <= item (prefix has V + 1 entries, prefix[0] = 0).
 __device__ __forceinline__ int find_view(const int* __restrict__ prefix, int V, int item) {
     int lo = 0, hi = V;
@@ -69,15 +65,10 @@ __global__ __launch_bounds__(256) void batch_sample_kernel(BatchHeader* __restri
     __shared__ int s_total;
     __shared__ int s_item;
     __shared__ int s_heavy[kMaxTileRays + 1];
-    __shared__ int s_prefix[kLdsViews + 1];
     const TileSmem sm = carve_tile_smem(s_dyn, tp, &s_total, s_heavy);
     const int n_items = h->items[kTile], V = h->V;
     const SampleOut out = h->out;
-    // the view prefix in LDS (its binary search runs once per tile)
-    const bool lds_prefix = V <= kLdsViews;
-    if (lds_prefix)
-        for (int i = threadIdx.x; i <= V; i += blockDim.x) s_prefix[i] = prefix[i];
-    const int* pf = lds_prefix ? s_prefix : prefix;
+    const int* pf = prefix;
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(&h->counter, 1);
         __syncthreads();

@@ -23,8 +23,9 @@
 #include "epcq_sampler.h"
 
 namespace epcq {
-// Defined in epcq_sample_kernels.h, instantiated in epcq_sample_inst_*.hip.
-template <int MODE, int KMAX, int SMAX>
+// Defined in epcq_sample_kernels.h, instantiated in epcq_sample_inst_*.hip
+// (NW = 1) and epcq_sample_wide_*.hip (NW = 4).
+template <int MODE, int KMAX, int SMAX, int NW>
 void launch_lay(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                 const TilePlan& tp, hipStream_t st);
 }  // namespace epcq
@@ -279,11 +280,19 @@ void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera*
     view.prad = prad;
 }
 
+// Windows of more than 64 lanes (k > 8) take the kernels with 4-word lane
+// sets (exact culling up to k = 16; larger windows visit every lane).
 template <int MODE, int KMAX>
 void launch_smax(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                  const TilePlan& tp, int bound, hipStream_t st) {
-    if (bound <= 2) launch_lay<MODE, KMAX, 2>(o, out, view, rays, R, tp, st);
-    else launch_lay<MODE, KMAX, kRegSamples>(o, out, view, rays, R, tp, st);
+    const bool wide = o.L > 64;
+    if (bound <= 2) {
+        if (wide) launch_lay<MODE, KMAX, 2, 4>(o, out, view, rays, R, tp, st);
+        else launch_lay<MODE, KMAX, 2, 1>(o, out, view, rays, R, tp, st);
+    } else {
+        if (wide) launch_lay<MODE, KMAX, kRegSamples, 4>(o, out, view, rays, R, tp, st);
+        else launch_lay<MODE, KMAX, kRegSamples, 1>(o, out, view, rays, R, tp, st);
+    }
 }
 
 // `bound`: samples of one ray that are stored (kernels with a bound of 2 keep

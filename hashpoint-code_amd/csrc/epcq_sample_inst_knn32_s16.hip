@@ -2,5 +2,5 @@
 #include "epcq_sample_kernels.h"
 
 namespace epcq {
-EPCQ_INSTANTIATE_LAUNCH(EPCQ_MODE_KNN, 32, kRegSamples)
+EPCQ_INSTANTIATE_LAUNCH(EPCQ_MODE_KNN, 32, kRegSamples, 1)
 }  // namespace epcq

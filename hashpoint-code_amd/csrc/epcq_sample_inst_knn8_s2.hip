@@ -2,5 +2,5 @@
 #include "epcq_sample_kernels.h"
 
 namespace epcq {
-EPCQ_INSTANTIATE_LAUNCH(EPCQ_MODE_KNN, 8, 2)
+EPCQ_INSTANTIATE_LAUNCH(EPCQ_MODE_KNN, 8, 2, 1)
 }  // namespace epcq

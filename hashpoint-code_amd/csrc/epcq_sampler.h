@@ -237,8 +237,81 @@ __device__ __forceinline__ float point_limit(const SampleOpts& o, const Src& src
     else return o.r2lim;
 }
 
-// Lane sets of one ray: bit l for window lane l (k*k lanes, at most 64).
-using LaneMask = uint64_t;
+// Lane sets of one ray: bit l for window lane l, in NW 64-bit words (NW = 1
+// for windows of at most 64 lanes, k <= 8; NW = 4 for up to 256, k <= 16).
+// Windows larger than 64 * NW lanes carry no sets: every lane is visited.
+// The lane -> (row, column) split l / k = (l * kdiv) >> 16 is exact for every
+// l < 256 and k <= 16 (the rounding of kdiv = ceil(65536 / k) adds less than
+// 256 / 65536 < 1 / k to the quotient).
+template <int NW>
+struct LaneSet {
+    uint64_t w[NW];
+
+    __device__ __forceinline__ static LaneSet none() {
+        LaneSet s;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) s.w[c] = 0ull;
+        return s;
+    }
+    __device__ __forceinline__ static LaneSet full() {
+        LaneSet s;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) s.w[c] = ~0ull;
+        return s;
+    }
+    __device__ __forceinline__ bool any() const {
+        uint64_t x = 0ull;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) x |= w[c];
+        return x != 0ull;
+    }
+    __device__ __forceinline__ LaneSet operator|(const LaneSet& b) const {
+        LaneSet s;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) s.w[c] = w[c] | b.w[c];
+        return s;
+    }
+    __device__ __forceinline__ LaneSet operator&(const LaneSet& b) const {
+        LaneSet s;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) s.w[c] = w[c] & b.w[c];
+        return s;
+    }
+    __device__ __forceinline__ LaneSet minus(const LaneSet& b) const {  // this & ~b
+        LaneSet s;
+#pragma unroll
+        for (int c = 0; c < NW; ++c) s.w[c] = w[c] & ~b.w[c];
+        return s;
+    }
+    // bits [lo, lo + n) |= bits of x (n <= 64), lo < 64 * NW
+    __device__ __forceinline__ void or_bits(uint64_t x, int lo) {
+        if constexpr (NW == 1) {
+            w[0] |= x << lo;
+        } else {
+            const int word = lo >> 6, sh = lo & 63;
+#pragma unroll
+            for (int c = 0; c < NW; ++c) {
+                if (c == word) w[c] |= x << sh;
+                if (c == word + 1 && sh > 0) w[c] |= x >> (64 - sh);
+            }
+        }
+    }
+    __device__ __forceinline__ void set(int l) { or_bits(1ull, l); }
+};
+
+// Calls fn(l) for the set lanes in order.
+template <int NW, class Fn>
+__device__ __forceinline__ void for_each_set_lane(const LaneSet<NW>& set, Fn&& fn) {
+#pragma unroll
+    for (int c = 0; c < NW; ++c) {
+        uint64_t m = set.w[c];
+        while (m) {
+            const int l = 64 * c + __ffsll((unsigned long long)m) - 1;
+            m &= m - 1ull;
+            fn(l);
+        }
+    }
+}
 
 // Calls fn(ub, vb, l) for the k*k neighbour lanes l in order: lane l is
 // (du, dv) = (l % k - (ko - 1), (l + k) / k - ko), i.e. rows dv = 1 - ko ..
@@ -257,45 +330,43 @@ __device__ __forceinline__ void for_each_lane(const SampleOpts& o, int u, int v,
     }
 }
 
-// Calls fn(ub, vb, l) for the lanes set in `mask`, in lane order.  Windows of
-// more than 64 lanes carry no masks: every lane is visited.
-template <class Fn>
-__device__ __forceinline__ void for_each_masked_lane(const SampleOpts& o, int u, int v, int W, int H, LaneMask mask,
-                                                     Fn&& fn) {
-    if (o.L > 64) {
+// Calls fn(ub, vb, l) for the lanes set in `mask`, in lane order (every lane
+// when the window has no lane sets).
+template <int NW, class Fn>
+__device__ __forceinline__ void for_each_masked_lane(const SampleOpts& o, int u, int v, int W, int H,
+                                                     const LaneSet<NW>& mask, Fn&& fn) {
+    if (o.L > 64 * NW) {
         for_each_lane(o, u, v, W, H, fn);
         return;
     }
-    while (mask) {
-        const int l = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1ull;
+    for_each_set_lane(mask, [&](int l) {
         const int r = (l * o.kdiv) >> 16;
         const int ub = min(max(u + (l - r * o.k) - (o.ko - 1), 0), W - 1);
         const int vb = min(max(v + r - (o.ko - 1), 0), H - 1);
         fn(ub, vb, l);
-    }
+    });
 }
 
 // Lanes whose bucket is non-empty (emask) and, of those, the ones whose cone
 // does not clear the ray by `margin` (returned; every lane when culling is
 // off).  Only the returned lanes can hold hits; `cand` counts their points
-// (0 for windows without masks).
-template <class Src>
-__device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                              int W, int H, float margin, LaneMask& emask, int& cand) {
+// (0 for windows without lane sets).
+template <int NW, class Src>
+__device__ __forceinline__ LaneSet<NW> lane_mask(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
+                                                 int W, int H, float margin, LaneSet<NW>& emask, int& cand) {
     cand = 0;
-    if (o.L > 64) {
-        emask = ~0ull;
-        return ~0ull;
+    if (o.L > 64 * NW) {
+        emask = LaneSet<NW>::full();
+        return LaneSet<NW>::full();
     }
-    LaneMask m = 0, e = 0;
+    LaneSet<NW> m = LaneSet<NW>::none(), e = LaneSet<NW>::none();
     int c = 0;
     for_each_lane(o, u, v, W, H, [&](int ub, int vb, int l) {
         const int n = src.bucket(ub, vb).y;
         if (n <= 0) return;
-        e |= 1ull << l;
+        e.set(l);
         if (o.cull && cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) return;
-        m |= 1ull << l;
+        m.set(l);
         c += n;
     });
     emask = e;
@@ -304,27 +375,29 @@ __device__ __forceinline__ LaneMask lane_mask(const SampleOpts& o, const Src& sr
 }
 
 // The lanes of `cand` whose cone does not clear the ray by `margin`.
-template <class Src>
-__device__ __forceinline__ LaneMask recheck_lanes(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                                  int W, int H, float margin, LaneMask cand) {
-    LaneMask m = 0;
+template <int NW, class Src>
+__device__ __forceinline__ LaneSet<NW> recheck_lanes(const SampleOpts& o, const Src& src, const Ray& ray, int u,
+                                                     int v, int W, int H, float margin, const LaneSet<NW>& cand) {
+    LaneSet<NW> m = LaneSet<NW>::none();
     for_each_masked_lane(o, u, v, W, H, cand, [&](int ub, int vb, int l) {
-        if (!cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) m |= 1ull << l;
+        if (!cone_clears(ray, src.cone4(ub, vb), src.cone2(ub, vb), margin)) m.set(l);
     });
     return m;
 }
 
 // Lanes whose unclamped window position lies in the image: one lane per
 // distinct window pixel.
-__device__ __forceinline__ LaneMask inside_lanes(const SampleOpts& o, int u, int v, int W, int H) {
-    LaneMask cols = 0, m = 0;
+template <int NW>
+__device__ __forceinline__ LaneSet<NW> inside_lanes(const SampleOpts& o, int u, int v, int W, int H) {
+    uint64_t cols = 0;
     for (int c = 0; c < o.k; ++c) {
         const int uu = u + c - (o.ko - 1);
         if (uu >= 0 && uu < W) cols |= 1ull << c;
     }
+    LaneSet<NW> m = LaneSet<NW>::none();
     for (int r = 0; r < o.k; ++r) {
         const int vv = v + r - (o.ko - 1);
-        if (vv >= 0 && vv < H) m |= cols << (r * o.k);
+        if (vv >= 0 && vv < H) m.or_bits(cols, r * o.k);
     }
     return m;
 }
@@ -334,9 +407,9 @@ __device__ __forceinline__ LaneMask inside_lanes(const SampleOpts& o, int u, int
 // order (quick_sampling_kernel :125-155), skipping the lanes off in `mask`
 // (empty or culled buckets, which hold no hit).  The next point is read
 // before the current one is tested.
-template <class Src, class Fn>
+template <int NW, class Src, class Fn>
 __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src& src, const Ray& ray, int u, int v,
-                                                 int W, int H, LaneMask mask, Fn&& fn) {
+                                                 int W, int H, const LaneSet<NW>& mask, Fn&& fn) {
     for_each_masked_lane(o, u, v, W, H, mask, [&](int ub, int vb, int) {
         const int2 e = src.bucket(ub, vb);
         if (e.y <= 0) return;
@@ -534,9 +607,10 @@ struct CompatStream {
 // mode walks the k*k lanes with their border duplicates, like
 // quick_search_nearby_pc (:39-47); KNN mode walks the distinct window pixels
 // (the caller restricts `mask` to inside_lanes).
-template <int MODE, int NS, class Src, class Buf>
+template <int MODE, int NS, int NW, class Src, class Buf>
 __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& src, int u, int v, int W, int H,
-                                               const float (&sp)[NS][3], int ns, LaneMask mask, Buf (&buf)[NS]) {
+                                               const float (&sp)[NS][3], int ns, const LaneSet<NW>& mask,
+                                               Buf (&buf)[NS]) {
     auto visit = [&](const float4& p, int jp) {
         const int id = __float_as_int(p.w);
         const float lim = point_limit(o, src, jp, p);
@@ -558,7 +632,7 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
             visit(p, e.x + j);
         }
     };
-    if (MODE == EPCQ_MODE_COMPAT || o.L <= 64) {
+    if (MODE == EPCQ_MODE_COMPAT || o.L <= 64 * NW) {
         for_each_masked_lane(o, u, v, W, H, mask, bucket);
     } else {
         const int ua = max(0, u - (o.ko - 1)), uz = min(W - 1, u + o.k - o.ko);
@@ -738,7 +812,7 @@ __device__ __forceinline__ float stored_sample_t(const SampleOpts& o, const Samp
 // With may_defer, a ray whose hit lanes hold more than o.heavy candidates is
 // left untouched and false returned: the caller then runs it with
 // run_ray_wave, which spreads its candidates over a whole wave.
-template <int MODE, int KMAX, int SMAX, int LAY, class Src>
+template <int MODE, int KMAX, int SMAX, int LAY, int NW, class Src>
 __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray,
                                         int u, int v, int W, int H, int64_t ri, bool valid_ray, bool may_defer) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;  // sample points served by one neighbour walk
@@ -748,9 +822,10 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
 #pragma unroll
     for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
     // lanes whose bucket may hold a hit (a ray with none is empty)
-    LaneMask emask = 0;
+    LaneSet<NW> emask = LaneSet<NW>::none();
     int cand = 0;
-    const LaneMask hmask = valid_ray ? lane_mask(o, src, ray, u, v, W, H, ray.omargin, emask, cand) : 0ull;
+    const LaneSet<NW> hmask =
+        valid_ray ? lane_mask<NW>(o, src, ray, u, v, W, H, ray.omargin, emask, cand) : LaneSet<NW>::none();
     if (may_defer && o.heavy > 0) {
         // only the wave's outliers: a ray above `heavy` while at most
         // kHeavyMinority of the wave's rays are, or one above 4x `heavy` (a
@@ -758,7 +833,7 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
         const int above = __popcll(__ballot(cand > o.heavy));
         if (cand > o.heavy && (above <= kHeavyMinority || cand > 4 * o.heavy)) return false;
     }
-    if (valid_ray && hmask != 0ull) {
+    if (valid_ray && hmask.any()) {
         // Step 1: tmin / tmax over the hits (WarpReduce min / max, :156-161).
         float tmin = 1e3f, tmax = -1e3f;
         for_each_ray_hit(o, src, ray, u, v, W, H, hmask, [&](float t, float) {
@@ -933,10 +1008,11 @@ __device__ __forceinline__ bool run_ray(const SampleOpts& o, const SampleOut& ou
             // of the ray and |sp - p| to a few ulp, so a bucket whose cone
             // clears the ray by that much more holds no neighbour.
             const float dl1 = fabsf(ray.d[0]) + fabsf(ray.d[1]) + fabsf(ray.d[2]);
-            LaneMask nmask = hmask;
-            if (o.cull && o.L <= 64)
-                nmask |= recheck_lanes(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1, emask & ~hmask);
-            if (MODE != EPCQ_MODE_COMPAT && o.L <= 64) nmask &= inside_lanes(o, u, v, W, H);
+            LaneSet<NW> nmask = hmask;
+            if (o.cull && o.L <= 64 * NW)
+                nmask = nmask | recheck_lanes<NW>(o, src, ray, u, v, W, H, ray.omargin + 6e-7f * tabs * dl1,
+                                                  emask.minus(hmask));
+            if (MODE != EPCQ_MODE_COMPAT && o.L <= 64 * NW) nmask = nmask & inside_lanes<NW>(o, u, v, W, H);
             neighbour_pass<MODE, NS>(o, src, u, v, W, H, pend, cnt, nmask, buf);
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
@@ -1079,34 +1155,54 @@ __device__ __forceinline__ void write_tail_wave(const SampleOpts& o, const Sampl
     }
 }
 
-template <int MODE, int KMAX, int SMAX, int LAY, class Src>
+template <int MODE, int KMAX, int SMAX, int LAY, int NW, class Src>
 __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
                              int W, int H, int64_t ri) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;
     const int lane = __lane_id();
-    // window lane `lane`: its bucket and cone test (same margins as run_ray)
-    int2 e = make_int2(0, -1);
-    int ubl = 0, vbl = 0;
-    bool nonempty = false, maybe = false;
-    if (lane < o.L) {
-        const int r = (lane * o.kdiv) >> 16;
-        ubl = min(max(u + (lane - r * o.k) - (o.ko - 1), 0), W - 1);
-        vbl = min(max(v + r - (o.ko - 1), 0), H - 1);
-        e = src.bucket(ubl, vbl);
-        nonempty = e.y > 0;
-        maybe = nonempty && !(o.cull && cone_clears(ray, src.cone4(ubl, vbl), src.cone2(ubl, vbl), ray.omargin));
+    // window lanes 64 c + lane (c < NW): their buckets and cone tests (same
+    // margins as run_ray)
+    int2 e[NW];
+    int ubl[NW], vbl[NW];
+    LaneSet<NW> emask, hmask;
+#pragma unroll
+    for (int c = 0; c < NW; ++c) {
+        const int l = 64 * c + lane;
+        e[c] = make_int2(0, -1);
+        ubl[c] = 0;
+        vbl[c] = 0;
+        bool nonempty = false, maybe = false;
+        if (l < o.L) {
+            const int r = (l * o.kdiv) >> 16;
+            ubl[c] = min(max(u + (l - r * o.k) - (o.ko - 1), 0), W - 1);
+            vbl[c] = min(max(v + r - (o.ko - 1), 0), H - 1);
+            e[c] = src.bucket(ubl[c], vbl[c]);
+            nonempty = e[c].y > 0;
+            maybe = nonempty &&
+                    !(o.cull && cone_clears(ray, src.cone4(ubl[c], vbl[c]), src.cone2(ubl[c], vbl[c]), ray.omargin));
+        }
+        emask.w[c] = __ballot(nonempty);
+        hmask.w[c] = __ballot(maybe);
     }
-    const LaneMask emask = __ballot(nonempty);
-    const LaneMask hmask = __ballot(maybe);
+
+    // Calls bucket_fn(ex, ey) on every lane for each window lane set in
+    // `set`, in lane order (the bucket's start and count, wave-uniform).
+    auto for_each_bucket = [&](const LaneSet<NW>& set, auto&& bucket_fn) {
+#pragma unroll
+        for (int c = 0; c < NW; ++c) {
+            uint64_t m = set.w[c];
+            while (m) {
+                const int x = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1ull;
+                bucket_fn(lane_bcast(e[c].x, x), lane_bcast(e[c].y, x));
+            }
+        }
+    };
 
     // fn(keep, t, e2) on every lane for each 64-candidate chunk of the hit
     // walk, `keep` set on the lanes holding a hit that the 48 cap keeps
     auto hit_walk = [&](auto&& fn) {
-        LaneMask m = hmask;
-        while (m) {
-            const int l = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1ull;
-            const int ex = lane_bcast(e.x, l), ey = lane_bcast(e.y, l);
+        for_each_bucket(hmask, [&](int ex, int ey) {
             int kept = 0;
             for (int c0 = 0; c0 < ey && kept < kSampleLaneHits; c0 += kWave) {
                 const int j = c0 + lane;
@@ -1123,14 +1219,14 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                 fn(keep, t, e2);
                 kept += __popcll(hb);
             }
-        }
+        });
     };
 
     int ns = 0;
     float ts_arr[SMAX];
 #pragma unroll
     for (int q = 0; q < SMAX; ++q) ts_arr[q] = 0.0f;
-    if (hmask != 0ull) {
+    if (hmask.any()) {
         // Step 1: tmin / tmax
         float tmin = 1e3f, tmax = -1e3f;
         hit_walk([&](bool keep, float t, float) {
@@ -1232,20 +1328,20 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                 buf[q].init();
             }
             const float dl1 = fabsf(ray.d[0]) + fabsf(ray.d[1]) + fabsf(ray.d[2]);
-            LaneMask nmask = hmask;
+            LaneSet<NW> nmask = hmask;
             if (o.cull) {
-                bool keep_lane = false;
-                if (((emask & ~hmask) >> lane) & 1ull)
-                    keep_lane = !cone_clears(ray, src.cone4(ubl, vbl), src.cone2(ubl, vbl),
-                                             ray.omargin + 6e-7f * tabs * dl1);
-                nmask |= __ballot(keep_lane);
+                const LaneSet<NW> culled = emask.minus(hmask);
+#pragma unroll
+                for (int c = 0; c < NW; ++c) {
+                    bool keep_lane = false;
+                    if ((culled.w[c] >> lane) & 1ull)
+                        keep_lane = !cone_clears(ray, src.cone4(ubl[c], vbl[c]), src.cone2(ubl[c], vbl[c]),
+                                                 ray.omargin + 6e-7f * tabs * dl1);
+                    nmask.w[c] |= __ballot(keep_lane);
+                }
             }
-            if (MODE != EPCQ_MODE_COMPAT) nmask &= inside_lanes(o, u, v, W, H);
-            LaneMask m = nmask;
-            while (m) {
-                const int l = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1ull;
-                const int ex = lane_bcast(e.x, l), ey = lane_bcast(e.y, l);
+            if (MODE != EPCQ_MODE_COMPAT) nmask = nmask & inside_lanes<NW>(o, u, v, W, H);
+            for_each_bucket(nmask, [&](int ex, int ey) {
                 for (int c0 = 0; c0 < ey; c0 += kWave) {
                     const int j = c0 + lane;
                     float dq[NS];
@@ -1279,7 +1375,7 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
                         }
                     }
                 }
-            }
+            });
             // feature rows are wide enough to share out; the other layouts
             // write from lane 0 (sharing them slows the one-lane form's code)
 #pragma unroll
@@ -1475,7 +1571,7 @@ __device__ __forceinline__ void defer_ray(int* heavy, int who) {
 // The block's deferred rays of a tile with top-left ray (u0, v0), each run by
 // one whole wave (run_ray_wave); waves take every nw-th entry.  The queue is
 // read after a barrier; the caller clears it before the next tile.
-template <int MODE, int KMAX, int SMAX, int LAY, class Src>
+template <int MODE, int KMAX, int SMAX, int LAY, int NW, class Src>
 __device__ __forceinline__ void run_deferred(const SampleOpts& o, const SampleOut& out, const ViewArgs& view,
                                              const Src& src, int tw, int u0, int v0, int64_t ray_off,
                                              const int* heavy) {
@@ -1486,7 +1582,7 @@ __device__ __forceinline__ void run_deferred(const SampleOpts& o, const SampleOu
         const int u = u0 + t % tw, v = v0 + t / tw;
         Ray ray;
         view_ray(view, u, v, ray);
-        run_ray_wave<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+        run_ray_wave<MODE, KMAX, SMAX, LAY, NW>(o, out, src, ray, u, v, view.W, view.H,
                                             ray_off + (int64_t)v * view.W + u);
     }
 }
@@ -1497,7 +1593,7 @@ __device__ __forceinline__ void run_deferred(const SampleOpts& o, const SampleOu
 // deferred and then run one per wave.
 // Returns false (block-uniform) when the tile's region did not fit the LDS;
 // the caller then samples the tile from global memory (sample_tile_global).
-template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR, int NW = 1>
 __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                  const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                  const TileSmem& sm) {
@@ -1516,12 +1612,12 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
     if (staged >= 0 && u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+        if (!run_ray<MODE, KMAX, SMAX, LAY, NW>(o, out, src, ray, u, v, view.W, view.H,
                                             ray_off + (int64_t)v * view.W + u, true, true))
             defer_ray(sm.heavy, (int)threadIdx.x);
     }
     __syncthreads();
-    if (staged >= 0) run_deferred<MODE, KMAX, SMAX, LAY>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
+    if (staged >= 0) run_deferred<MODE, KMAX, SMAX, LAY, NW>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
     __syncthreads();  // the LDS is reused by the block's next tile
     return staged >= 0;
 }
@@ -1530,7 +1626,7 @@ __device__ __forceinline__ bool sample_tile_body(const SampleOpts& o0, const Sam
 // after sample_tile_body returned false, so the LDS still holds the halo's
 // hash entries (not rebased) and cones, and its deferred-ray queue is empty.
 // Points are read from global memory.  Block-wide.
-template <int MODE, int KMAX, int SMAX, int LAY, bool PR>
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR, int NW = 1>
 __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const SampleOut& out, const ViewArgs& view,
                                                    const TilePlan& tp, int tile, int tiles_x, int64_t ray_off,
                                                    const TileSmem& sm) {
@@ -1546,12 +1642,12 @@ __device__ __forceinline__ void sample_tile_global(const SampleOpts& o0, const S
     if (u < view.W && v < view.H) {
         Ray ray;
         view_ray(view, u, v, ray);
-        if (!run_ray<MODE, KMAX, SMAX, LAY>(o, out, src, ray, u, v, view.W, view.H,
+        if (!run_ray<MODE, KMAX, SMAX, LAY, NW>(o, out, src, ray, u, v, view.W, view.H,
                                             ray_off + (int64_t)v * view.W + u, true, true))
             defer_ray(sm.heavy, (int)threadIdx.x);
     }
     __syncthreads();
-    run_deferred<MODE, KMAX, SMAX, LAY>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
+    run_deferred<MODE, KMAX, SMAX, LAY, NW>(o, out, view, src, tp.tw, u0, v0, ray_off, sm.heavy);
 }
 
 // Cone of pixel p's points seen from the view origin o (see cone_gap).  The

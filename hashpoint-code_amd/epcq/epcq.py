@@ -96,7 +96,8 @@ class NeuralPoints(nn.Module):
 
     pc_type 'pc' wraps the given `points` [N,3]; 'pointnerf' loads a Point-NeRF
     checkpoint (tensors only, torch.load(weights_only=True)); 'gt_pc' reads a
-    point cloud file with open3d when that package is installed."""
+    PLY point cloud (with open3d when that package is installed, else with
+    epcq.io.read_ply, which returns the same arrays)."""
 
     def __init__(self,
                  basis_dim: int = 9,
@@ -130,11 +131,16 @@ class NeuralPoints(nn.Module):
             import numpy as np
             try:
                 import open3d as o3d
-            except ImportError as e:
-                raise ImportError("pc_type='gt_pc' needs open3d to read " + str(gt_pc_path)) from e
-            pcd = o3d.io.read_point_cloud(gt_pc_path)
-            pts = torch.tensor(np.asarray(pcd.points), dtype=torch.float32, device=device)
-            cols = torch.tensor(np.asarray(pcd.colors), dtype=torch.float32, device=device)
+            except ImportError:
+                o3d = None
+            if o3d is not None:
+                pcd = o3d.io.read_point_cloud(gt_pc_path)
+                pts_np, cols_np = np.asarray(pcd.points), np.asarray(pcd.colors)
+            else:  # the same arrays from the package's own PLY reader
+                from .io import read_ply
+                pts_np, cols_np = read_ply(gt_pc_path)
+            pts = torch.tensor(pts_np, dtype=torch.float32, device=device)
+            cols = torch.tensor(cols_np, dtype=torch.float32, device=device)
             self.basis_dim = basis_dim
             self.xyz_data = nn.Parameter(pts, requires_grad=False)
             self.rgb_data = nn.Parameter(cols, requires_grad=True)

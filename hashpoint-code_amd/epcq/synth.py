@@ -267,11 +267,21 @@ def workload_batched(n=1_000_000, n_views=64):
     return pts, cams, dict(radius_threshold=0.005, num_point_cloud_per_sp=8, topK=8)
 
 
+def window_for_radius(r_max, focal, distance):
+    """Window size k (odd) whose half-width covers the largest radius r_max
+    projected at the viewing distance: k = 2 ceil(r_max f / distance) + 1."""
+    return 2 * int(math.ceil(r_max * focal / distance)) + 1
+
+
 def workload_large(n=20_000_000, n_views=16):
-    """20M points on 200 boxes + plane, per-point radius, 16 views 1920x1080."""
+    """20M points on 200 boxes + plane, per-point radius, 16 views 1920x1080;
+    the window is sized from the largest radius at the eye distance 4
+    (k = 15 for r_max = 0.01, f = 2666)."""
     pts = box_scene(n, n_boxes=200, seed=30, jitter=0.002)
     radius = log_uniform_radius(n, seed=31)
     dirs = Stream(32, 4).normal(3 * n_views).reshape(n_views, 3)
     dirs = dirs / np.sqrt((dirs * dirs).sum(axis=1))[:, None]
-    cams = [look_at_camera(4.0 * d, 1920, 1080, nerf_focal(1920)) for d in dirs]
-    return pts, radius, cams, dict(radius_threshold=0.01, num_point_cloud_per_sp=16, topK=16)
+    f = nerf_focal(1920)
+    cams = [look_at_camera(4.0 * d, 1920, 1080, f) for d in dirs]
+    k = window_for_radius(0.01, f, 4.0)
+    return pts, radius, cams, dict(radius_threshold=0.01, num_point_cloud_per_sp=16, topK=16, kernel_size=k)

@@ -16,16 +16,19 @@
 // Stand-ins for the sampler launch templates (instantiated in their own units
 // in the library); the checks below return before any launch.
 namespace epcq {
-template <int MODE, int KMAX, int SMAX>
+template <int MODE, int KMAX, int SMAX, int NW>
 void launch_lay(const SampleOpts&, const SampleOut&, const ViewArgs&, const int*, int64_t, const TilePlan&,
                 hipStream_t) {
     std::abort();
 }
+#define EPCQ_STAND_IN_NW(MODE, KMAX, NW)                                                                          \
+    template void launch_lay<MODE, KMAX, 2, NW>(const SampleOpts&, const SampleOut&, const ViewArgs&,            \
+                                                const int*, int64_t, const TilePlan&, hipStream_t);              \
+    template void launch_lay<MODE, KMAX, kRegSamples, NW>(const SampleOpts&, const SampleOut&, const ViewArgs&,  \
+                                                          const int*, int64_t, const TilePlan&, hipStream_t);
 #define EPCQ_STAND_IN(MODE, KMAX)                                                                                 \
-    template void launch_lay<MODE, KMAX, 2>(const SampleOpts&, const SampleOut&, const ViewArgs&, const int*,    \
-                                            int64_t, const TilePlan&, hipStream_t);                              \
-    template void launch_lay<MODE, KMAX, kRegSamples>(const SampleOpts&, const SampleOut&, const ViewArgs&,      \
-                                                      const int*, int64_t, const TilePlan&, hipStream_t);        \
+    EPCQ_STAND_IN_NW(MODE, KMAX, 1)                                                                              \
+    EPCQ_STAND_IN_NW(MODE, KMAX, 4)                                                                              \
     template <>                                                                                                  \
     void launch_batch_sample<MODE, KMAX, 2>(int, const TilePlan&, BatchHeader*, const BatchView*, const int*,     \
                                             const SampleOpts&, hipStream_t) {                                    \

@@ -1,6 +1,7 @@
 """The large configuration at full size: 20M points on 200 boxes and the
 ground plane, per-point radius LogUniform[0.001, 0.01], one 1920x1080 view
-of the 16, K = 16, 32-dim bf16 features.  The hash table is compared with the
+of the 16, a window of k = 15 sized from the largest radius (225 neighbour
+pixels, culled with 4-word lane sets), K = 16, 32-dim bf16 features.  The hash table is compared with the
 oracle in full; the sampling on a random set of rays (the oracle's cost) and,
 over the whole image, through properties that hold at any size and the
 agreement of the tiled and ray-list samplers."""
