@@ -95,31 +95,29 @@ __global__ __launch_bounds__(kBlock) void sh_mark_kernel(const int* __restrict__
     if (i == n - 1 || keys[i + 1] != key) hash[2 * (int64_t)key + 1] = (int)(i + 1);
 }
 
+// Run ends become counts; with `sort` each run is ordered by depth, short
+// runs by one thread, long ones by the block (scratch (sd, sx)[n]).
 __global__ __launch_bounds__(kBlock) void sh_fix_kernel(int* __restrict__ hash, int P, int sort,
-                                                        int* __restrict__ out_idx,
-                                                        float* __restrict__ out_depth) {
+                                                        int* __restrict__ out_idx, float* __restrict__ out_depth,
+                                                        float* __restrict__ sd, int* __restrict__ sx) {
+    __shared__ int s_cnt[kBlock];
+    __shared__ int s_end[kBlock];
+    __shared__ int s_wave[kBuildWaves];
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= P) return;
-    const int s = hash[2 * p];
-    if (s < 0) return;
-    const int e = hash[2 * p + 1];
-    const int c = e - s;
-    hash[2 * p + 1] = c;
-    if (!sort || c < 2) return;
-    float* d = out_depth + s;
-    int* ix = out_idx + s;
-    for (int a = 1; a < c; ++a) {
-        const float kd = d[a];
-        const int ki = ix[a];
-        int j = a - 1;
-        while (j >= 0 && d[j] > kd) {
-            d[j + 1] = d[j];
-            ix[j + 1] = ix[j];
-            --j;
+    int c = 0, e = 0;
+    if (p < P) {
+        const int s = hash[2 * p];
+        if (s >= 0) {
+            e = hash[2 * p + 1];
+            c = e - s;
+            hash[2 * p + 1] = c;
         }
-        d[j + 1] = kd;
-        ix[j + 1] = ki;
     }
+    if (!sort) return;
+    s_cnt[threadIdx.x] = c;
+    s_end[threadIdx.x] = e;
+    __syncthreads();
+    sort_pixel_runs(kBlock, s_cnt, s_end, 0, out_depth, out_idx, sd, sx, s_wave);
 }
 
 // ---- scatter (per-pixel B nearest) -------------------------------------------
@@ -315,21 +313,32 @@ extern "C" int epcq_build_hashtable_noalloc(const float* xyz, int64_t n, const f
                                 kNoallocScratchBytes, stream, who);
 }
 
+extern "C" size_t epcq_scatter_hashtable_workspace_size(int64_t n, int32_t sort) {
+    if (n <= 0 || !sort) return 0;
+    return (size_t)n * (sizeof(float) + sizeof(int32_t));
+}
+
 extern "C" int epcq_scatter_hashtable(const int32_t* point_idx, const int32_t* keys, const float* depth,
                                       int64_t n, int32_t height, int32_t width, int32_t sort,
-                                      int32_t* out_idx, float* out_depth, int32_t* out_hash, void* stream) {
+                                      int32_t* out_idx, float* out_depth, int32_t* out_hash, void* workspace,
+                                      size_t workspace_bytes, void* stream) {
     if (height <= 0 || width <= 0) return fail(EPCQ_ERR_INVALID, "scatter_hashtable: image size must be positive");
     if (n < 0 || n >= (int64_t(1) << 31)) return fail(EPCQ_ERR_INVALID, "scatter_hashtable: point count out of range");
     const int64_t P = (int64_t)height * width;
     if (P >= (int64_t(1) << 31) / 2) return fail(EPCQ_ERR_INVALID, "scatter_hashtable: image too large");
+    const size_t need = epcq_scatter_hashtable_workspace_size(n, sort);
+    if (workspace_bytes < need || (need && !workspace))
+        return fail(EPCQ_ERR_WORKSPACE, "scatter_hashtable: workspace too small");
     hipStream_t st = as_stream(stream);
     if (int e = fill_launch(out_hash, 2 * P, -1, st)) return e;
     if (n == 0) return EPCQ_OK;
     sh_mark_kernel<<<dim3((unsigned)ceil_div(n, kBlock)), dim3(kBlock), 0, st>>>(point_idx, keys, depth, n, (int)P,
                                                                                 out_idx, out_depth, out_hash);
     if (int e = check_launch("scatter_hashtable/mark")) return e;
+    float* sd = need ? static_cast<float*>(workspace) : nullptr;
+    int* sx = need ? reinterpret_cast<int*>(static_cast<float*>(workspace) + n) : nullptr;
     sh_fix_kernel<<<dim3((unsigned)ceil_div(P, kBlock)), dim3(kBlock), 0, st>>>(out_hash, (int)P, sort, out_idx,
-                                                                               out_depth);
+                                                                               out_depth, sd, sx);
     return check_launch("scatter_hashtable/fix");
 }
 

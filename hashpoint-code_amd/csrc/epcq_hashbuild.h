@@ -141,6 +141,116 @@ __device__ __forceinline__ int ordered_append(int* cursor, uint32_t key, int bit
 }
 
 
+// ---- depth order of a pixel run (the reference's stable bubble sort,
+// cuda_util.cuh:291-313, with its strict '>' comparison) ------------------------
+
+// Unsigned key with the order of the float comparison: negative values
+// reversed, and -0 equal to +0.  Depths of on-image points are finite (a
+// non-finite depth makes the screen coordinate NaN or infinite).
+__device__ __forceinline__ uint32_t depth_key(float d) {
+    const uint32_t u = d == 0.0f ? 0u : __float_as_uint(d);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+constexpr int kSmallRun = 32;  // runs up to this length are sorted by one thread
+
+// Stable insertion sort of a short run by one thread.
+__device__ __forceinline__ void thread_sort_run(float* d, int* ix, int c) {
+    for (int a = 1; a < c; ++a) {
+        const float kd = d[a];
+        const int ki = ix[a];
+        int j = a - 1;
+        while (j >= 0 && d[j] > kd) {
+            d[j + 1] = d[j];
+            ix[j + 1] = ix[j];
+            --j;
+        }
+        d[j + 1] = kd;
+        ix[j + 1] = ki;
+    }
+}
+
+// Stable sort of a long run (d, ix)[0, c) by the whole block: a least-
+// significant-digit radix sort of depth_key, 8 bits a pass, through the
+// scratch (sd, sx)[0, c); a pass whose digit is the same for the whole run
+// moves nothing.  O(c) per pass where the insertion sort is O(c^2).
+// s_dig: 256 LDS cursors.  Every thread of the block calls it with the same c.
+__device__ void block_sort_run(float* d, int* ix, float* sd, int* sx, int c, int* s_dig, int* s_wave) {
+    float* src_d = d;
+    int* src_i = ix;
+    float* dst_d = sd;
+    int* dst_i = sx;
+    for (int shift = 0; shift < 32; shift += 8) {
+        for (int k = threadIdx.x; k < 256; k += kBuildBlock) s_dig[k] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < c; i += kBuildBlock) atomicAdd(&s_dig[(depth_key(src_d[i]) >> shift) & 255u], 1);
+        __syncthreads();
+        const bool uniform = s_dig[(depth_key(src_d[0]) >> shift) & 255u] == c;
+        __syncthreads();
+        if (uniform) continue;
+        lds_exclusive_scan(s_dig, 256, s_wave);
+        for (int r0 = 0; r0 < c; r0 += kBuildBlock) {
+            const int i = r0 + threadIdx.x;
+            const bool active = i < c;
+            const float v = active ? src_d[i] : 0.0f;
+            const int id = active ? src_i[i] : 0;
+            const uint32_t dig = active ? (depth_key(v) >> shift) & 255u : 0u;
+            const int pos = ordered_append(s_dig, dig, 8, active);
+            if (active) {
+                dst_d[pos] = v;
+                dst_i[pos] = id;
+            }
+        }
+        __syncthreads();
+        float* td = src_d;
+        src_d = dst_d;
+        dst_d = td;
+        int* ti = src_i;
+        src_i = dst_i;
+        dst_i = ti;
+    }
+    if (src_d != d) {
+        for (int i = threadIdx.x; i < c; i += kBuildBlock) {
+            d[i] = src_d[i];
+            ix[i] = src_i[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Sorts the pixel runs of `tile` pixels by depth: run f has count cnt[f] and
+// ends at base + end[f] of (d, ix); (sd, sx) are scratch at the same
+// positions.  Short runs one per thread, long runs by the whole block, one
+// after another in pixel order.
+__device__ void sort_pixel_runs(int tile, const int* cnt, const int* end, int64_t base, float* d, int* ix,
+                                float* sd, int* sx, int* s_wave) {
+    __shared__ int s_dig[256];
+    __shared__ uint64_t s_hot[kBuildWaves];
+    for (int f = threadIdx.x; f < tile; f += kBuildBlock) {
+        const int c = cnt[f];
+        if (c < 2 || c > kSmallRun) continue;
+        const int64_t s = base + end[f] - c;
+        thread_sort_run(d + s, ix + s, c);
+    }
+    for (int f0 = 0; f0 < tile; f0 += kBuildBlock) {
+        const int f = f0 + threadIdx.x;
+        const uint64_t m = __ballot(f < tile && cnt[f] > kSmallRun);
+        if (__lane_id() == 0) s_hot[threadIdx.x / kWave] = m;
+        __syncthreads();
+        for (int w = 0; w < kBuildWaves; ++w) {
+            uint64_t mm = s_hot[w];
+            while (mm) {
+                const int g = f0 + w * kWave + (__ffsll((unsigned long long)mm) - 1);
+                mm &= mm - 1;
+                const int c = cnt[g];
+                const int64_t s = base + end[g] - c;
+                block_sort_run(d + s, ix + s, sd + s, sx + s, c, s_dig, s_wave);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Everything one view's build reads and writes.
 struct BuildView {
     const float* xyz;
@@ -410,27 +520,9 @@ __device__ __forceinline__ void fine_body(const BuildView& bv, int b, int* s_mem
     }
     if (bv.sort && out_depth) {
         __syncthreads();
-        // stable insertion sort of each pixel run by depth (same result as the
-        // reference's stable bubble sort, cuda_util.cuh:291-313)
-        for (int f = threadIdx.x; f < tile; f += kBuildBlock) {
-            const int c = s_cnt[f];
-            if (c < 2) continue;
-            const int s = bs + s_cur[f] - c;
-            float* d = out_depth + s;
-            int* ix = bv.out_idx + s;
-            for (int a = 1; a < c; ++a) {
-                const float kd = d[a];
-                const int ki = ix[a];
-                int j = a - 1;
-                while (j >= 0 && d[j] > kd) {
-                    d[j + 1] = d[j];
-                    ix[j + 1] = ix[j];
-                    --j;
-                }
-                d[j + 1] = kd;
-                ix[j + 1] = ki;
-            }
-        }
+        // each pixel run by depth; the binned index and depth arrays of this
+        // bin are free once placed and serve as the long runs' scratch
+        sort_pixel_runs(tile, s_cnt, s_cur, bs, out_depth, bv.out_idx, bv.tdep, bv.tidx, s_wave);
     }
     // off-image tail: positions [N_in, n) hold -1 like the reference's
     // torch::full(-1) outputs; bin b fills every nbins-th run of 256

@@ -90,7 +90,8 @@ ABI_SYMBOLS = {
     "epcq_set_tuning": (_I32, [ctypes.POINTER(_Tuning)]),
     "epcq_get_tuning": (_I32, [ctypes.POINTER(_Tuning)]),
     "epcq_squared_threshold": (_F32, [_F32]),
-    "epcq_scatter_hashtable": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "epcq_scatter_hashtable_workspace_size": (_SZ, [_I64, _I32]),
+    "epcq_scatter_hashtable": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "epcq_scatter": (_I32, [_P, _P, _P, _I64, _I32, _I32, _I32, _P, _P, _P]),
     "epcq_build_hashtable_workspace_size": (_SZ, [_I64, _I32, _I32]),
     "epcq_build_hashtable": (_I32, [_P, _I64, _P, _P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
@@ -316,10 +317,14 @@ def scatter_hashtable(point_idx_list, point2img_idx_list, depth_list, img_size, 
     out_idx = torch.empty(n, dtype=torch.int32, device=dev)
     out_depth = torch.empty(n, dtype=torch.float32, device=dev)
     out_hash = torch.empty((H * W, 2), dtype=torch.int32, device=dev)
+    sort = int(bool(sort))
+    ws_bytes = _lib.epcq_scatter_hashtable_workspace_size(n, sort)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
     with torch.cuda.device(dev):
         _raise_on(_lib.epcq_scatter_hashtable(_ptr(point_idx_list), _ptr(point2img_idx_list), _ptr(depth_list),
-                                              n, H, W, int(bool(sort)), _ptr(out_idx), _ptr(out_depth),
-                                              _ptr(out_hash), _stream(dev)))
+                                              n, H, W, sort, _ptr(out_idx), _ptr(out_depth),
+                                              _ptr(out_hash), _ptr(ws) if ws is not None else None, ws_bytes,
+                                              _stream(dev)))
     return out_idx, out_depth, out_hash
 
 

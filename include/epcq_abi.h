@@ -101,12 +101,16 @@ float epcq_squared_threshold(float r);
  * Inputs are pixel-sorted (keys ascending, off-image key = H*W) as produced
  * by epcq.py:283-286.  Writes every element of out_idx[n], out_depth[n]
  * (-1 outside pixel runs) and out_hash[H*W*2] ((start, count) or (-1, -1));
- * with sort != 0 each run is ordered by depth ascending, stable.
+ * with sort != 0 each run is ordered by depth ascending, stable (runs longer
+ * than 32 points by a block-wide radix sort through the workspace).
+ * workspace: device memory of at least
+ * epcq_scatter_hashtable_workspace_size(n, sort) bytes (0 when sort == 0).
  */
+size_t epcq_scatter_hashtable_workspace_size(int64_t n, int32_t sort);
 int epcq_scatter_hashtable(const int32_t* point_idx, const int32_t* keys, const float* depth,
                            int64_t n, int32_t height, int32_t width, int32_t sort,
                            int32_t* out_idx, float* out_depth, int32_t* out_hash,
-                           void* stream);
+                           void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * scatter -- replaces `scatter` (epcq/csrc/epcq.cpp:5,

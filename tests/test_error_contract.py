@@ -83,8 +83,12 @@ def test_workspace_and_size_checks():
     assert lib.epcq_build_hashtable_workspace_size(-1, 8, 8) == 0
     rc = lib.epcq_scatter(None, None, None, 10, 4, 4, 0, None, None, None)
     assert rc == 1 and "bins must be positive" in _err(mod)
-    rc = lib.epcq_scatter_hashtable(None, None, None, -5, 4, 4, 0, None, None, None, None)
+    rc = lib.epcq_scatter_hashtable(None, None, None, -5, 4, 4, 0, None, None, None, None, 0, None)
     assert rc == 1 and "point count out of range" in _err(mod)
+    need = lib.epcq_scatter_hashtable_workspace_size(1000, 1)
+    assert need == 8000 and lib.epcq_scatter_hashtable_workspace_size(1000, 0) == 0
+    rc = lib.epcq_scatter_hashtable(None, None, None, 1000, 4, 4, 1, None, None, None, None, need - 1, None)
+    assert rc == 3 and _err(mod) == "scatter_hashtable: workspace too small"
     cam = _cam(mod)
     cam.c2w = ctypes.c_void_p(8)
     rc = lib.epcq_quick_query(None, 0, ctypes.byref(cam), ctypes.byref(_opts(mod)), None, None, None, -1, None, None)

@@ -69,6 +69,68 @@ def test_build_edge_cases(dev):
     check_build(mix, cam, dev, sort=False)
 
 
+def long_runs(n, levels, offsets=(0.0,)):
+    """n points on the optical axis of a camera at (0, -4, 0), depths from 7
+    down to -3 in point order (the insertion sort's worst case; negative depths
+    are points behind the camera, which project onto the centre pixel too),
+    rounded to 1/levels so that many depths repeat; point i is shifted
+    sideways by offsets[i % len(offsets)] to spread the runs over pixels."""
+    y = np.round(np.linspace(3.0, -7.0, n) * levels) / levels
+    pts = np.zeros((n, 3), np.float32)
+    pts[:, 1] = y
+    pts[:, 0] = np.asarray(offsets, np.float32)[np.arange(n) % len(offsets)]
+    return pts
+
+
+def test_depth_sort_long_runs_vs_oracle(dev):
+    """Runs far longer than one thread sorts (32 points) go through the block-
+    wide radix sort: the fused build and scatter_hashtable against the oracle."""
+    import epcq.csrc as C
+    cam = synth.look_at_camera([0.0, -4.0, 0.0], 64, 48, 80.0)
+    pts = long_runs(24_000, 100, offsets=(0.0, 0.0, 0.03, -0.05))
+    print("depth sort: fused build, runs of ~12k", flush=True)
+    gi, gd, gh = check_build(pts, cam, dev, sort=True)
+    assert gh[:, 1].max() > 5000 and (gd[gd != -1] < 0).any()
+    print("depth sort: scatter_hashtable", flush=True)
+    ix, ks, ds = presorted_inputs(pts, cam)
+    gi, gd, gh = [t.cpu().numpy() for t in C.scatter_hashtable(to_dev(ix, dev), to_dev(ks, dev), to_dev(ds, dev),
+                                                             (cam.height, cam.width), True)]
+    oi, od, oh = oracle.scatter_hashtable(ix, ks, ds, cam.height, cam.width, True)
+    np.testing.assert_array_equal(gh, oh)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(bits(gd), bits(od))
+
+
+def test_depth_sort_hot_pixel_properties(dev):
+    """300k points in (almost) one pixel, depths descending in point order
+    with ties: each run equals the stable argsort of the oracle's projected
+    depths (the oracle's bubble sort would take minutes at this size)."""
+    import epcq.csrc as C
+    cam = synth.look_at_camera([0.0, -4.0, 0.0], 64, 48, 80.0)
+    pts = long_runs(300_000, 1000)
+    key, depth = oracle.project(pts, cam.w2c, cam.K, cam.height, cam.width)
+    P = cam.height * cam.width
+    inside = np.nonzero(key < P)[0]
+    print("depth sort: 300k points on the optical axis", flush=True)
+    gi, gd, gh = gpu_build(pts, cam, dev, sort=True)
+    assert gh[:, 1].max() > 250_000
+    for p in np.unique(key[inside]):
+        sel = inside[key[inside] == p]
+        want = sel[np.argsort(depth[sel], kind="stable")]
+        s, c = gh[p]
+        assert c == sel.size
+        np.testing.assert_array_equal(gi[s:s + c], want)
+        np.testing.assert_array_equal(bits(gd[s:s + c]), bits(depth[want]))
+    assert (gi[inside.size:] == -1).all()
+    print("depth sort: scatter_hashtable, the same runs", flush=True)
+    ix, ks, ds = presorted_inputs(pts, cam)
+    si, sd, sh = [t.cpu().numpy() for t in C.scatter_hashtable(to_dev(ix, dev), to_dev(ks, dev), to_dev(ds, dev),
+                                                             (cam.height, cam.width), True)]
+    np.testing.assert_array_equal(si, gi)
+    np.testing.assert_array_equal(bits(sd), bits(gd))
+    np.testing.assert_array_equal(sh, gh)
+
+
 def test_build_deterministic(dev):
     pts, cam = synth.box_scene(500_000, seed=9), synth.look_at_camera([2.0, 2.0, 2.0], 640, 512, 888.8)
     a = gpu_build(pts, cam, dev)

@@ -152,6 +152,25 @@ def test_scatter_hashtable_reference_semantics_on_presorted_input():
     np.testing.assert_array_equal(oh[5], [5, 1])
 
 
+def test_scatter_hashtable_sort_long_runs_negative_depths():
+    """The depth sort of a long run equals a stable argsort by depth: depths
+    descending in input order (the bubble sort's worst case), many ties, and
+    negative depths (points behind the camera project into the image too)."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    keys = np.sort(rng.integers(0, 3, size=n)).astype(np.int32)
+    depth = np.round(np.linspace(6.0, -3.0, n, dtype=np.float32) * 20) / 20
+    depth = depth.astype(np.float32)
+    pidx = rng.permutation(n).astype(np.int32)
+    oi, od, oh = oracle.scatter_hashtable(pidx, keys, depth, 1, 3, sort=True)
+    for p in range(3):
+        s, c = oh[p]
+        sel = np.nonzero(keys == p)[0]
+        order = sel[np.argsort(depth[sel], kind="stable")]
+        np.testing.assert_array_equal(oi[s:s + c], pidx[order])
+        np.testing.assert_array_equal(od[s:s + c], depth[order])
+
+
 def test_scatter_top_b_brute_force():
     rng = np.random.default_rng(3)
     H, W, B = 3, 4, 3
