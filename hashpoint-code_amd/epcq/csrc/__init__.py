@@ -77,6 +77,14 @@ class _BatchViewOut(ctypes.Structure):
                 ("nb_idx", ctypes.c_void_p), ("nb_t", ctypes.c_void_p), ("nb_dist", ctypes.c_void_p)]
 
 
+_AGG_FIELDS = ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4", "wa", "ba", "c1", "cb1", "c2", "cb2", "c3", "cb3",
+               "c4", "cb4")
+
+
+class _AggParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in _AGG_FIELDS]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -109,6 +117,9 @@ ABI_SYMBOLS = {
     "epcq_max_live_samples": (_I32, [ctypes.POINTER(_Options), _F32]),
     "epcq_idw_features": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
     "epcq_debug_expf": (_I32, [_P, _P, _I64, _P]),
+    "epcq_aggregator_workspace_size": (_SZ, [_I64]),
+    "epcq_aggregator_forward": (_I32, [_P, _P, _P, _P, _P, _P, _P, _I64, _I32, _I32, ctypes.POINTER(_AggParams), _P, _P,
+                                       _P, _SZ, _P]),
     "epcq_clamp_t": (_I32, [_P, _I64, _I32, _F32, _P, _P]),
     "epcq_render_rays": (_I32, [_P, _P, _F32, _P, _P, _I64, _I32, _I32, _P, _P]),
     "epcq_render_rays_backward": (_I32, [_P, _P, _F32, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P]),
@@ -685,6 +696,63 @@ def render_rays_backward(sigmas, rgbs, grad_out, white_bg=False, deltas=None, t_
                                                  _ptr(grad_out), R, S, int(bool(white_bg)), _ptr(gs), _ptr(gc),
                                                  _stream(dev)))
     return gs, gc
+
+
+# --------------------------------------------------------------- aggregator
+# (in_features padded to a multiple of 16, stored as bf16) of each bf16 layer
+_AGG_PAD = {"w1": 288, "w2": 256, "w3": 272, "w4": 256, "c1": 288, "c2": 128, "c3": 128}
+
+
+def pack_aggregator(layers):
+    """Device parameters of the Aggregator kernels from its Linear layers,
+    given as {name: (weight [out, in], bias [out])} with the names of
+    epcq_aggregator_params (w1..w4 block1/block3, wa alpha_branch, c1..c4
+    color_branch).  Returns (struct, tensors): the tensors own the memory the
+    struct points to."""
+    keep = {}
+    for name, (w, b) in layers.items():
+        w = w.detach()
+        b = b.detach()
+        if name in _AGG_PAD:
+            pw = torch.zeros((w.shape[0], _AGG_PAD[name]), dtype=torch.bfloat16, device=w.device)
+            pw[:, :w.shape[1]] = w.to(torch.bfloat16)
+        else:
+            pw = w.to(torch.float32).reshape(-1)
+        keep[name] = pw.contiguous()
+        keep["b" + name[1:] if name[0] == "w" else "cb" + name[1:]] = b.to(torch.float32).reshape(-1).contiguous()
+    st = _AggParams(**{n: ctypes.c_void_p(keep[n].data_ptr()) for n in _AGG_FIELDS})
+    return st, keep
+
+
+def aggregator_forward(packed, pc_xyz, pc_rgb, pc_dir, pc_feat, sp_xyz, valid, rays_d):
+    """Aggregator.forward (epcq/epcq.py:439-507) on the compacted samples:
+    pc_xyz / pc_rgb / pc_dir [M,K,3], pc_feat [M,K,32], sp_xyz [M,3],
+    valid [M,K] bool, rays_d [M,3] -> (alpha [M], rgb [M,3]) float32."""
+    st, _ = packed
+    for t, n in ((pc_xyz, "pc_xyz"), (pc_rgb, "pc_rgb"), (pc_dir, "pc_dir"), (pc_feat, "pc_feat"),
+                 (sp_xyz, "sp_xyz"), (rays_d, "rays_d")):
+        _check_input(t, n)
+        _check_dtype(t, torch.float32, n)
+    _check_input(valid, "valid")
+    _check_dtype(valid, torch.bool, "valid")
+    M, K = valid.shape
+    F = pc_feat.shape[-1]
+    for t, n, shape in ((pc_xyz, "pc_xyz", (M, K, 3)), (pc_rgb, "pc_rgb", (M, K, 3)), (pc_dir, "pc_dir", (M, K, 3)),
+                        (pc_feat, "pc_feat", (M, K, F)), (sp_xyz, "sp_xyz", (M, 3)), (rays_d, "rays_d", (M, 3))):
+        if tuple(t.shape) != shape:
+            raise RuntimeError(f"{n} must have shape {list(shape)}, got {list(t.shape)}")
+    dev = valid.device
+    _check_same_device(dev, ("pc_xyz", pc_xyz), ("pc_rgb", pc_rgb), ("pc_dir", pc_dir), ("pc_feat", pc_feat),
+                       ("sp_xyz", sp_xyz), ("rays_d", rays_d))
+    alpha = torch.empty(M, dtype=torch.float32, device=dev)
+    rgb = torch.empty((M, 3), dtype=torch.float32, device=dev)
+    ws_bytes = _lib.epcq_aggregator_workspace_size(M)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev) if ws_bytes else None
+    with torch.cuda.device(dev):
+        _raise_on(_lib.epcq_aggregator_forward(_ptr(pc_xyz), _ptr(pc_rgb), _ptr(pc_dir), _ptr(pc_feat), _ptr(sp_xyz),
+                                               _ptr(valid), _ptr(rays_d), M, K, F, ctypes.byref(st), _ptr(alpha),
+                                               _ptr(rgb), _ptr(ws), ws_bytes, _stream(dev)))
+    return alpha, rgb
 
 
 def debug_expf(x):

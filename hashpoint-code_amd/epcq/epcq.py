@@ -9,9 +9,11 @@ gfx950 build (projection, pixel keys, stable sort and hash table in one
 pipeline) instead of torch matmuls + torch.sort + scatter_hashtable, with
 pixel lists in the deterministic (pixel, index) order.  The volume-rendering
 epilogue of the reference module (clamp_t :508-515, render :517-530) runs on
-gfx950 kernels too, with a backward pass for training; the Aggregator MLP and
-positional encoding between sampling and rendering are not part of this
-package.
+gfx950 kernels too, with a backward pass for training.  The Aggregator
+(:406-507) between them keeps the reference's layers and parameter names and
+runs its inference forward on the gfx950 matrix cores (bf16 inputs and
+weights, fp32 accumulation); positional_encoding (:389-404) is the torch
+form the reference uses.
 """
 from dataclasses import dataclass
 from typing import Optional, Union
@@ -278,6 +280,90 @@ def calculate_weight(dist: torch.Tensor, mask: torch.Tensor = None):
     if mask is not None:
         inv = inv * mask.float()
     return inv / (inv.sum(dim=-1, keepdim=True) + 1e-6)
+
+
+def positional_encoding(positions: torch.Tensor, freqs: int, ori: bool = False):
+    """[sin(x_i 2^j), cos(x_i 2^j)] pairs over inputs i and frequencies
+    j < freqs, i major (epcq.py:389-404); with ori, [x, sin(all), cos(all)]."""
+    bands = 2.0 ** torch.arange(freqs, dtype=torch.float32, device=positions.device)
+    D = positions.shape[-1]
+    pts = (positions[..., None] * bands).reshape(*positions.shape[:-1], freqs * D)
+    if ori:
+        return torch.cat([positions, torch.sin(pts), torch.cos(pts)], dim=-1)
+    return torch.stack([torch.sin(pts), torch.cos(pts)], dim=-1).reshape(*pts.shape[:-1], 2 * freqs * D)
+
+
+class Aggregator(nn.Module):
+    """Point aggregator of the reference renderer (epcq.py:406-507) with the
+    same layers, initialisation and parameter names, so its state dicts load
+    as they are.  forward takes the FeatureQuery outputs of the live samples
+    (pc_xyz / pc_rgb / pc_dir [M,K,3], pc_feat [M,K,32], output_sp_xyz [M,3],
+    valid_sp_mask_out [M,K], rays_d [M,3]) and returns alphas and colours
+    scattered to mask_out's shape: [H*W, S] and [H*W, S, 3] for an image mask
+    [H,W,S], [R, S] and [R, S, 3] for a ray mask [R,S].
+
+    The forward runs on gfx950 (epcq_aggregator_forward): the point MLP of
+    every (sample, neighbour) pair and the colour MLP on the bf16 matrix
+    cores with fp32 accumulation, the inverse-distance aggregation in fp32;
+    results agree with the fp32 module within bf16 rounding.  It is an
+    inference forward: with gradients enabled on the parameters it raises
+    (training is not part of this package)."""
+
+    def __init__(self) -> None:
+        super().__init__()
+        self.block1 = nn.Sequential(nn.Linear(284, 256), nn.LeakyReLU(inplace=True),
+                                    nn.Linear(256, 256), nn.LeakyReLU(inplace=True))
+        self.block3 = nn.Sequential(nn.Linear(263, 256), nn.LeakyReLU(inplace=True),
+                                    nn.Linear(256, 256), nn.LeakyReLU(inplace=True))
+        self.alpha_branch = nn.Sequential(nn.Linear(256, 1))
+        self.alpha_act = nn.Softplus()
+        self.color_branch = nn.Sequential(nn.Linear(280, 128), nn.LeakyReLU(inplace=True),
+                                          nn.Linear(128, 128), nn.LeakyReLU(inplace=True),
+                                          nn.Linear(128, 128), nn.LeakyReLU(inplace=True),
+                                          nn.Linear(128, 3))
+        self.color_act = nn.Sigmoid()
+        for mod in self.modules():
+            if isinstance(mod, nn.Linear):
+                nn.init.xavier_uniform_(mod.weight)
+                nn.init.constant_(mod.bias, 0.0)
+        self.chunksize = 20000  # kept for compatibility; the kernels need no chunking
+        self._packed = None
+        self._packed_key = None
+
+    def calculate_weight(self, dist, mask=None):
+        return calculate_weight(dist, mask)
+
+    def _layers(self):
+        lin = lambda m: (m.weight, m.bias)  # noqa: E731
+        return {"w1": lin(self.block1[0]), "w2": lin(self.block1[2]), "w3": lin(self.block3[0]),
+                "w4": lin(self.block3[2]), "wa": lin(self.alpha_branch[0]), "c1": lin(self.color_branch[0]),
+                "c2": lin(self.color_branch[2]), "c3": lin(self.color_branch[4]), "c4": lin(self.color_branch[6])}
+
+    def packed_params(self):
+        """bf16 / f32 device copies of the parameters, rebuilt when a
+        parameter changes (its storage or version)."""
+        key = tuple((p.data_ptr(), p._version, p.device) for p in self.parameters())
+        if self._packed is None or key != self._packed_key:
+            self._packed = _C.pack_aggregator(self._layers())
+            self._packed_key = key
+        return self._packed
+
+    def forward(self, pc_xyz, pc_rgb, pc_dir, pc_feat, output_sp_xyz, valid_sp_mask_out, mask_out, rays_d):
+        if torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters()):
+            raise RuntimeError("Aggregator: the gfx950 kernels implement the inference forward; "
+                               "call it under torch.no_grad()")
+        if mask_out.dim() not in (2, 3):
+            raise ValueError("mask_out shape error")
+        alpha, rgb = _C.aggregator_forward(self.packed_params(), pc_xyz.contiguous(), pc_rgb.contiguous(),
+                                           pc_dir.contiguous(), pc_feat.contiguous(), output_sp_xyz.contiguous(),
+                                           valid_sp_mask_out.contiguous(), rays_d.contiguous())
+        live = mask_out.reshape(-1)
+        rows = mask_out.shape[0] * mask_out.shape[1] if mask_out.dim() == 3 else mask_out.shape[0]
+        S = mask_out.shape[-1]
+        fa = torch.zeros(live.numel(), dtype=torch.float32, device=alpha.device).masked_scatter_(live, alpha)
+        fc = torch.zeros((live.numel(), 3), dtype=torch.float32, device=rgb.device)
+        fc.masked_scatter_(live[:, None].expand(-1, 3), rgb)
+        return fa.view(rows, S), fc.view(rows, S, 3)
 
 
 def clamp_t(t_val: torch.Tensor, vsize: float = 0.004):

@@ -314,6 +314,39 @@ int epcq_batch_launch(epcq_batch* b, void* stream);   /* replay the graph */
 int epcq_batch_profile(epcq_batch* b, void* stream, float* stage_ms);
 void epcq_batch_destroy(epcq_batch* b);
 
+/*
+ * aggregator_forward -- replaces Aggregator.forward (epcq/epcq.py:439-507)
+ * for inference: block1 and block3 over every valid (sample, neighbour) pair,
+ * alpha_branch with softplus(x - 1), the inverse-distance aggregation of the
+ * pair alphas and features (calculate_weight, :434-437), and color_branch
+ * with sigmoid(x) * 1.002 - 0.001, on bf16 matrix cores with fp32
+ * accumulation.  M samples, K in [1, 32] neighbours, F = 32 features:
+ * pc_xyz / pc_rgb / pc_dir [M,K,3], pc_feat [M,K,F], sp_xyz [M,3],
+ * valid [M,K] (bool bytes), rays_d [M,3], all f32 unless stated ->
+ * out_alpha [M], out_rgb [M,3].  params: the module's Linear layers as bf16
+ * [out, in] row-major matrices with the inputs zero-padded to a multiple of
+ * 16 (w1 256x288, w2 256x256, w3 256x272, w4 256x256, c1 128x288, c2 and c3
+ * 128x128) and f32 biases; alpha_branch (wa [256], ba [1]) and the last
+ * colour layer (c4 [3,128], cb4 [3]) stay f32.  workspace: 16-byte aligned,
+ * at least epcq_aggregator_workspace_size(M) bytes.
+ */
+typedef struct epcq_aggregator_params {
+    const void* w1;  const float* b1;   /* block1.0: 284 -> 256 */
+    const void* w2;  const float* b2;   /* block1.2: 256 -> 256 */
+    const void* w3;  const float* b3;   /* block3.0: 263 -> 256 */
+    const void* w4;  const float* b4;   /* block3.2: 256 -> 256 */
+    const float* wa; const float* ba;   /* alpha_branch.0: 256 -> 1 */
+    const void* c1;  const float* cb1;  /* color_branch.0: 280 -> 128 */
+    const void* c2;  const float* cb2;  /* color_branch.2: 128 -> 128 */
+    const void* c3;  const float* cb3;  /* color_branch.4: 128 -> 128 */
+    const float* c4; const float* cb4;  /* color_branch.6: 128 -> 3 */
+} epcq_aggregator_params;
+size_t epcq_aggregator_workspace_size(int64_t M);
+int epcq_aggregator_forward(const float* pc_xyz, const float* pc_rgb, const float* pc_dir, const float* pc_feat,
+                            const float* sp_xyz, const uint8_t* valid, const float* rays_d, int64_t M, int32_t K,
+                            int32_t F, const epcq_aggregator_params* params, float* out_alpha, float* out_rgb,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
 /* Test hook: our exp() on the device, out[i] = exp(in[i]). */
 int epcq_debug_expf(const float* in, float* out, int64_t n, void* stream);
 
