@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched", choices=["batched", "serving", "box", "sphere", "large"])
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "serving", "box", "sphere", "large", "aggregator"])
     ap.add_argument("--mode", default=None, choices=["compat", "knn"],
                     help="neighbour mode (default: compat, knn for the large workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -602,6 +603,94 @@ def cpu_baseline_views(wl, opt, target_s, mode, reps=5):
             "per_class": per_class}
 
 
+# algorithmic FLOPs of the Aggregator: per valid (sample, neighbour) pair the
+# four point layers and the alpha head, per sample the colour branch
+AGG_FLOP_PAIR = 2 * (284 * 256 + 256 * 256 + 263 * 256 + 256 * 256 + 256)
+AGG_FLOP_SAMPLE = 2 * (280 * 128 + 128 * 128 + 128 * 128 + 128 * 3)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_aggregator(args):
+    """Aggregator forward (the reference renderer's MLP after sampling) on
+    the live samples of an 800x800 view with two samples per ray: 1,280,000
+    samples, K = 8 neighbours (90 % valid), 32-dim features; bf16 matrix
+    cores with fp32 accumulation.  Beside it the fp32 torch formulation of
+    the reference module on the same GPU (rocBLAS GEMMs) and on the host."""
+    import numpy as np
+    import torch
+
+    import epcq.csrc as C
+    from epcq.epcq import Aggregator
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from aggregator_ref import aggregator_reference, random_inputs
+
+    dev = torch.device("cuda:0")
+    M, K = 1_280_000, 8
+    inputs = random_inputs(M, K, seed=1, p_valid=0.9, empty_every=0)
+    dinp = [t.to(dev) for t in inputs]
+    pairs = int(inputs[5].sum())
+    torch.manual_seed(0)
+    mod = Aggregator().to(dev)
+    stream = torch.cuda.current_stream(dev)
+    with torch.no_grad():
+        pk = mod.packed_params()
+        for _ in range(args.warmup):
+            C.aggregator_forward(pk, *dinp)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(stream)
+            C.aggregator_forward(pk, *dinp)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+        # the fp32 torch formulation of the reference module on this GPU
+        ref_ms = []
+        for i in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            aggregator_reference(mod, *dinp)
+            torch.cuda.synchronize()
+            ref_ms.append((time.perf_counter() - t1) * 1e3)
+        # and on the host, on a sample of the reference's chunk size
+        ncpu = 20000
+        cpu_inp = [t[:ncpu] for t in inputs]
+        mod_cpu = Aggregator()
+        mod_cpu.load_state_dict({k: v.cpu() for k, v in mod.state_dict().items()})
+        aggregator_reference(mod_cpu, *cpu_inp)
+        cpu_t = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            aggregator_reference(mod_cpu, *cpu_inp)
+            cpu_t.append(time.perf_counter() - t1)
+    avg = float(np.mean(ms))
+    flops = pairs * AGG_FLOP_PAIR + M * AGG_FLOP_SAMPLE
+    ach = flops / (avg * 1e-3) / 1e12
+    log(f"aggregator: {avg:.3f} ms per forward, {M / avg * 1e3 / 1e6:.1f} M samples/s, {ach:.1f} TFLOP/s")
+    return {
+        "metric": "samples/s aggregator forward (point + colour MLPs, IDW aggregation)",
+        "value": M / (avg * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": avg, "wall_s": wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "aggregator-1.28M-samples-K8", "samples": M, "K": K, "valid_pairs": pairs,
+                   "features": 32, "layers": "284-256-256 | 263-256-256 | 256-1 | 280-128-128-128-3"},
+        "roofline": {"bound": "mfma", "kernel": "agg_pair_kernel + agg_color_kernel",
+                     "achieved": ach, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "flops_model": "valid pairs x 2(284x256 + 256x256 + 263x256 + "
+                     "256x256 + 256) + samples x 2(280x128 + 2x128x128 + 128x3)"},
+        "torch_fp32_gpu": {"ms_per_step": float(np.median(ref_ms)), "value": M / (float(np.median(ref_ms)) * 1e-3),
+                           "unit": "samples/s", "speedup": float(np.median(ref_ms)) / avg},
+        "cpu_baseline": {"value": ncpu / float(np.median(cpu_t)), "unit": "samples/s",
+                         "cores": torch.get_num_threads(), "kind": "port", "cpu_model": cpu_model(),
+                         "sample": f"{ncpu} samples (the reference's chunk size), fp32 torch formulation of the "
+                                   f"reference module, median of 3 after a warm-up"},
+    }
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of `mib` MiB, five times: the reference
     point that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -628,6 +717,9 @@ def main():
         sys.exit(2)
     if args.workload == "serving":
         print(json.dumps(serving_record(args)), flush=True)
+        return
+    if args.workload == "aggregator":
+        print(json.dumps(bench_aggregator(args)), flush=True)
         return
     wl = load_workload(args.workload)
     if args.workload == "batched":

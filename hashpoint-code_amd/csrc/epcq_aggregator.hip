@@ -43,16 +43,34 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kWave = 64;
 constexpr int kStrip = 32;              // rows per wave
-constexpr int kWavesPerBlock = 2;
+constexpr int kWavesPerBlock = 4;          // colour kernel
 constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kPairWaves = 8;              // pair kernel: 256 rows share each weight chunk
+constexpr int kPairBlock = kWave * kPairWaves;
 constexpr int kLdsStride = 296;         // bf16 per LDS row (288 + 8: rows start on different banks)
 constexpr int kFeat = 32;               // F: block1 takes 7F + 60 = 284 inputs
 constexpr int kX1 = 288, kH = 256, kX3 = 272, kXc = 288, kC = 128;
 
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    const uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
-    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+// Round to nearest even (the hardware conversion, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    return (uint32_t)to_bf16(lo) | ((uint32_t)to_bf16(hi) << 16);
+}
+
+// sin and cos of x for the positional encodings, whose results are rounded
+// to bf16: one range reduction to [-1/2, 1/2] revolution and the hardware
+// sine and cosine (absolute error ~1e-6 for |x| < 1024, far below the bf16
+// step), the library's sincosf beyond.
+__device__ __forceinline__ void pe_sincos(float x, float& s, float& c) {
+    if (fabsf(x) < 1024.0f) {
+        float r = x * 0.15915494309189535f;
+        r = r - rintf(r);
+        s = __builtin_amdgcn_sinf(r);
+        c = __builtin_amdgcn_cosf(r);
+    } else {
+        sincosf(x, &s, &c);
+    }
 }
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -86,13 +104,27 @@ __device__ __forceinline__ void strip_layer(uint16_t* io, const uint16_t* __rest
     for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
     const uint16_t* a_row = io + row * kLdsStride + kh;
     const uint16_t* w_row = W + (size_t)row * K + kh;
-#pragma unroll 2
+    // the weight operands of steps ks + 1 and ks + 2 load while step ks
+    // multiplies (they come from L2: two steps cover its latency)
+    bf16x8 b0[NT], b1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        b0[t] = load8(w_row + (size_t)t * 32 * K);
+        b1[t] = load8(w_row + (size_t)t * 32 * K + (KS > 1 ? 16 : 0));
+    }
+#pragma unroll 1
     for (int ks = 0; ks < KS; ++ks) {
         const bf16x8 a = load8(a_row + ks * 16);
+        const int kn = (ks + 2 < KS ? ks + 2 : KS - 1) * 16;
+        bf16x8 bn[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bn[t] = load8(w_row + (size_t)t * 32 * K + kn);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[t], acc[t], 0, 0, 0);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = load8(w_row + (size_t)t * 32 * K + ks * 16);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+            b0[t] = b1[t];
+            b1[t] = bn[t];
         }
     }
     const int r0 = (lane >> 5) * 4;
@@ -106,6 +138,60 @@ __device__ __forceinline__ void strip_layer(uint16_t* io, const uint16_t* __rest
             float v = acc[t][r] + bc;
             if (LEAKY) v = lrelu(v);
             io[rr * kLdsStride + c] = to_bf16(v);
+        }
+    }
+}
+
+// strip_layer for the pair kernel's 256-wide layers, with the weight rows
+// staged through LDS for the whole block: step ks of the K loop multiplies
+// the chunk W[:, 16 ks, 16 ks + 16) (8 KB), which the block's 512 threads
+// copy from L2 into s_w, 16 bytes each, two steps ahead through registers.
+// Every wave of the block calls it (it holds block barriers); `active`
+// waves multiply, the others only help with the copies.
+template <int KS>
+__device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const uint16_t* __restrict__ W,
+                                            const float* __restrict__ bias, bool active) {
+    constexpr int K = 16 * KS;
+    constexpr int NT = kH / 32;
+    static_assert(KS >= 2, "two chunks in flight");
+    const int tid = threadIdx.x;
+    const int lane = tid % kWave;
+    const int row = lane & 31;
+    const int kh = (lane >> 5) * 8;
+    const uint16_t* wsrc = W + (size_t)(tid >> 1) * K + (tid & 1) * 8;
+    uint4 c0, c1;
+    __builtin_memcpy(&c0, wsrc, 16);
+    __builtin_memcpy(&c1, wsrc + 16, 16);
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+    const uint16_t* a_row = io + row * kLdsStride + kh;
+#pragma unroll 1
+    for (int ks = 0; ks < KS; ++ks) {
+        __syncthreads();  // the previous chunk has been read
+        __builtin_memcpy(s_w + tid * 8, &c0, 16);
+        c0 = c1;
+        if (ks + 2 < KS) __builtin_memcpy(&c1, wsrc + (ks + 2) * 16, 16);
+        __syncthreads();
+        if (active) {
+            const bf16x8 a = load8(a_row + ks * 16);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 b = load8(s_w + (t * 32 + row) * 16 + kh);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    if (!active) return;
+    const int r0 = (lane >> 5) * 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int c = t * 32 + row;
+        const float bc = bias[c];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rr = (r >> 2) * 8 + r0 + (r & 3);
+            io[rr * kLdsStride + c] = to_bf16(lrelu(acc[t][r] + bc));
         }
     }
 }
@@ -125,13 +211,17 @@ struct PairArgs {
     uint16_t* xc;          // [M, 288] bf16 colour-branch inputs
 };
 
-__global__ __launch_bounds__(kBlock) void agg_pair_kernel(PairArgs a) {
-    __shared__ __align__(16) uint16_t s_rows[kWavesPerBlock][kStrip * kLdsStride];
+__global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
+    __shared__ __align__(16) uint16_t s_rows[kPairWaves][kStrip * kLdsStride];
+    __shared__ __align__(16) uint16_t s_w[kH * 16];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     uint16_t* io = s_rows[wave];
     const int Kp = 1 << a.kp_log2;
-    const int64_t row0 = ((int64_t)blockIdx.x * kWavesPerBlock + wave) * kStrip;
+    const int64_t rows_total = (int64_t)a.M << a.kp_log2;
+    if ((int64_t)blockIdx.x * kPairWaves * kStrip >= rows_total) return;  // whole block past the end
+    const int64_t row0 = ((int64_t)blockIdx.x * kPairWaves + wave) * kStrip;
+    const bool live_strip = row0 < rows_total;  // waves past the end still take part in the copies
     const int row = lane & 31;
     const int half = lane >> 5;
     const int64_t r = row0 + row;
@@ -140,7 +230,6 @@ __global__ __launch_bounds__(kBlock) void agg_pair_kernel(PairArgs a) {
     const bool in_range = m < a.M;
     const bool valid = in_range && k < a.K && a.valid[m * a.K + k] != 0;
     const int64_t pair = m * a.K + k;
-    if (row0 >= (int64_t)a.M << a.kp_log2) return;  // whole strip past the end (uniform per wave)
 
     // ---- x1 rows: lane half 0 takes features 0..15, half 1 features 16..31
     // and the distance terms; invalid rows are zero
@@ -156,36 +245,52 @@ __global__ __launch_bounds__(kBlock) void agg_pair_kernel(PairArgs a) {
         pz = a.pc_xyz[3 * pair + 2];
     }
     uint16_t* xr = io + row * kLdsStride;
-    for (int d = half * 16; d < half * 16 + 16; ++d) {
-        const float f = valid ? a.pc_feat[pair * kFeat + d] : 0.0f;
-        xr[d] = to_bf16(f);
+    float4 fv[4];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const float x = f * (float)(1 << j);
-            xr[kFeat + 2 * (d * 3 + j)] = to_bf16(sinf(x));
-            xr[kFeat + 2 * (d * 3 + j) + 1] = to_bf16(cosf(x));
-        }
+    for (int q = 0; q < 4; ++q)
+        fv[q] = valid ? reinterpret_cast<const float4*>(a.pc_feat + pair * kFeat + half * 16)[q]
+                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float* fs = reinterpret_cast<const float*>(fv);
+    {
+        uint4 w0 = make_uint4(pack_bf16(fs[0], fs[1]), pack_bf16(fs[2], fs[3]), pack_bf16(fs[4], fs[5]),
+                              pack_bf16(fs[6], fs[7]));
+        uint4 w1 = make_uint4(pack_bf16(fs[8], fs[9]), pack_bf16(fs[10], fs[11]), pack_bf16(fs[12], fs[13]),
+                              pack_bf16(fs[14], fs[15]));
+        __builtin_memcpy(xr + half * 16, &w0, 16);
+        __builtin_memcpy(xr + half * 16 + 8, &w1, 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        // PE(f, 3) of feature d = half * 16 + i: 6 values at 32 + 6 d
+        float sn[3], cs[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) pe_sincos(fs[i] * (float)(1 << j), sn[j], cs[j]);
+        uint32_t* o = reinterpret_cast<uint32_t*>(xr + kFeat + 6 * (half * 16 + i));
+        o[0] = pack_bf16(sn[0], cs[0]);
+        o[1] = pack_bf16(sn[1], cs[1]);
+        o[2] = pack_bf16(sn[2], cs[2]);
     }
     if (half) {
         // dist = p - s and distpow = p^2 - s^2; invalid pairs have p = 0 as
         // the reference's zero-filled gathers
         const float df[6] = {px - sx, py - sy, pz - sz, px * px - sx * sx, py * py - sy * sy, pz * pz - sz * sz};
+        uint32_t* o = reinterpret_cast<uint32_t*>(xr + 224);
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
-                const float x = df[i] * (float)(1 << j);
-                xr[224 + 2 * (i * 5 + j)] = to_bf16(sinf(x));
-                xr[224 + 2 * (i * 5 + j) + 1] = to_bf16(cosf(x));
+                float sn, cs;
+                pe_sincos(df[i] * (float)(1 << j), sn, cs);
+                o[i * 5 + j] = pack_bf16(sn, cs);
             }
         }
-#pragma unroll
-        for (int c = 284; c < kX1; ++c) xr[c] = 0;
+        o[30] = 0;  // columns 284..287
+        o[31] = 0;
     }
     const bool any_valid = __ballot(valid) != 0;
+    block_layer<kX1 / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w1), a.p.b1, any_valid);
+    block_layer<kH / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w2), a.p.b2, any_valid);
     if (any_valid) {
-        strip_layer<kX1 / 16, kH / 32, true>(io, static_cast<const uint16_t*>(a.p.w1), a.p.b1);
-        strip_layer<kH / 16, kH / 32, true>(io, static_cast<const uint16_t*>(a.p.w2), a.p.b2);
         // x3 = [h2, rgb, dir - d, dir . d, 0...]
         if (half == 0) {
             float rgb[3] = {0.0f, 0.0f, 0.0f}, dir[3] = {0.0f, 0.0f, 0.0f}, rd[3] = {0.0f, 0.0f, 0.0f};
@@ -214,15 +319,29 @@ __global__ __launch_bounds__(kBlock) void agg_pair_kernel(PairArgs a) {
 #pragma unroll
             for (int c = kH + 7; c < kX3; ++c) xr[c] = 0;
         }
-        strip_layer<kX3 / 16, kH / 32, true>(io, static_cast<const uint16_t*>(a.p.w3), a.p.b3);
-        strip_layer<kH / 16, kH / 32, true>(io, static_cast<const uint16_t*>(a.p.w4), a.p.b4);
-    } else {
-        for (int c = half * 128; c < half * 128 + 128; ++c) xr[c] = 0;
     }
+    block_layer<kX3 / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w3), a.p.b3, any_valid);
+    block_layer<kH / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w4), a.p.b4, any_valid);
+    if (!live_strip) return;
+    if (!any_valid)
+        for (int c = half * 128; c < half * 128 + 128; ++c) xr[c] = 0;
 
     // ---- per-pair alpha: lane halves take 128 columns each
     float part = 0.0f;
-    for (int c = half * 128; c < half * 128 + 128; ++c) part += a.p.wa[c] * from_bf16(xr[c]);
+    {
+        const float4* wa4 = reinterpret_cast<const float4*>(a.p.wa + half * 128);
+        const uint16_t* hr = xr + half * 128;
+#pragma unroll 8
+        for (int q = 0; q < 32; ++q) {
+            const float4 wq = wa4[q];
+            uint2 h;
+            __builtin_memcpy(&h, hr + 4 * q, sizeof(h));
+            part += wq.x * __uint_as_float(h.x << 16);
+            part += wq.y * __uint_as_float(h.x & 0xffff0000u);
+            part += wq.z * __uint_as_float(h.y << 16);
+            part += wq.w * __uint_as_float(h.y & 0xffff0000u);
+        }
+    }
     const float other = __shfl_xor(part, 32);
     const float dotv = (half ? other + part : part + other) + a.p.ba[0];
     const float alpha_k = softplus(dotv - 1.0f);
@@ -267,9 +386,9 @@ __global__ __launch_bounds__(kBlock) void agg_pair_kernel(PairArgs a) {
         __builtin_memcpy(xo + lane * 4, &packed, sizeof(packed));
         if (lane < 12) {
             const int i = lane / 4, j = lane % 4;
-            const float x = a.rays_d[3 * ms + i] * (float)(1 << j);
-            xo[kH + 2 * lane] = to_bf16(sinf(x));
-            xo[kH + 2 * lane + 1] = to_bf16(cosf(x));
+            float sn, cs;
+            pe_sincos(a.rays_d[3 * ms + i] * (float)(1 << j), sn, cs);
+            reinterpret_cast<uint32_t*>(xo + kH)[lane] = pack_bf16(sn, cs);
         } else if (lane < 20) {
             xo[kH + 24 + (lane - 12)] = 0;
         }
@@ -306,10 +425,21 @@ __global__ __launch_bounds__(kBlock) void agg_color_kernel(ColorArgs a) {
     strip_layer<kC / 16, kC / 32, true>(io, static_cast<const uint16_t*>(a.p.c3), a.p.cb3);
     // last layer (128 -> 3) in fp32, halves take 64 columns each
     float part[3] = {0.0f, 0.0f, 0.0f};
-    for (int c = half * 64; c < half * 64 + 64; ++c) {
-        const float h = from_bf16(xr[c]);
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+        const int c = half * 64 + 4 * q;
+        uint2 h;
+        __builtin_memcpy(&h, xr + c, sizeof(h));
+        const float hv[4] = {__uint_as_float(h.x << 16), __uint_as_float(h.x & 0xffff0000u),
+                             __uint_as_float(h.y << 16), __uint_as_float(h.y & 0xffff0000u)};
 #pragma unroll
-        for (int j = 0; j < 3; ++j) part[j] += a.p.c4[j * kC + c] * h;
+        for (int j = 0; j < 3; ++j) {
+            const float4 wq = *reinterpret_cast<const float4*>(a.p.c4 + j * kC + c);
+            part[j] += wq.x * hv[0];
+            part[j] += wq.y * hv[1];
+            part[j] += wq.z * hv[2];
+            part[j] += wq.w * hv[3];
+        }
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -361,9 +491,9 @@ extern "C" int epcq_aggregator_forward(const float* pc_xyz, const float* pc_rgb,
     pa.out_alpha = out_alpha;
     pa.xc = static_cast<uint16_t*>(workspace);
     const int64_t strips = ceil_div(M << pa.kp_log2, kStrip);
-    const int64_t blocks = ceil_div(strips, kWavesPerBlock);
+    const int64_t blocks = ceil_div(strips, kPairWaves);
     if (blocks > INT32_MAX) return fail(EPCQ_ERR_INVALID, "aggregator: too many samples");
-    agg_pair_kernel<<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(pa);
+    agg_pair_kernel<<<dim3((unsigned)blocks), dim3(kPairBlock), 0, st>>>(pa);
     if (int e = check_launch("aggregator/pairs")) return e;
     ColorArgs ca;
     ca.xc = pa.xc;
