@@ -717,9 +717,10 @@ def pack_aggregator(layers):
             pw = torch.zeros((w.shape[0], _AGG_PAD[name]), dtype=torch.bfloat16, device=w.device)
             pw[:, :w.shape[1]] = w.to(torch.bfloat16)
         else:
-            pw = w.to(torch.float32).reshape(-1)
+            pw = w.to(torch.float32).reshape(-1).clone()
+        # fresh allocations: the kernels read weights and biases 16 bytes at a time
         keep[name] = pw.contiguous()
-        keep["b" + name[1:] if name[0] == "w" else "cb" + name[1:]] = b.to(torch.float32).reshape(-1).contiguous()
+        keep["b" + name[1:] if name[0] == "w" else "cb" + name[1:]] = b.to(torch.float32).reshape(-1).clone()
     st = _AggParams(**{n: ctypes.c_void_p(keep[n].data_ptr()) for n in _AGG_FIELDS})
     return st, keep
 
