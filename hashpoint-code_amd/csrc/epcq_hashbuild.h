@@ -269,7 +269,7 @@ struct BuildView {
     int* tidx;    // [n] binned point index (null when tpt is used)
     float4* tpt;  // [n] binned (x, y, z, index bits) records, or null
     float* tdep;  // [n] binned depth
-    uint16_t* tfine;  // [n] pixel within bin
+    uint16_t* tfine;  // [n] pixel within bin (null with tpt: recomputed from the record)
     int* out_idx;     // [n]
     float* out_depth; // [n] (may be null: not written)
     int* out_hash;    // [P, 2]
@@ -323,7 +323,7 @@ inline BuildPlan plan_build(int64_t n, int H, int W, bool points = false) {
     bp.off_tidx = take((size_t)n * (points ? 16 : 4));
     bp.off_tdep = take(points ? 0 : (size_t)n * 4);
     bp.points = points;
-    bp.off_tfine = take((size_t)n * 2);
+    bp.off_tfine = take(points ? 0 : (size_t)n * 2);  // records recompute their pixel
     bp.bytes = off;
     return bp;
 }
@@ -355,7 +355,7 @@ inline BuildView make_build_view(const BuildPlan& bp, const float* xyz, int64_t 
     v.tidx = bp.points ? nullptr : reinterpret_cast<int*>(w + bp.off_tidx);
     v.tpt = bp.points ? reinterpret_cast<float4*>(w + bp.off_tidx) : nullptr;
     v.tdep = bp.points ? nullptr : reinterpret_cast<float*>(w + bp.off_tdep);
-    v.tfine = reinterpret_cast<uint16_t*>(w + bp.off_tfine);
+    v.tfine = bp.points ? nullptr : reinterpret_cast<uint16_t*>(w + bp.off_tfine);
     v.out_idx = out_idx;
     v.out_depth = out_depth;
     v.out_hash = out_hash;
@@ -469,7 +469,7 @@ __device__ __forceinline__ void bin_body(const BuildView& bv, int blk, int* s_cu
             if (bv.tpt) bv.tpt[pos] = make_float4(x, y, z, __int_as_float((int)i));
             else bv.tidx[pos] = (int)i;
             if (bv.out_depth) bv.tdep[pos] = depth;  // depths only when they are an output
-            bv.tfine[pos] = (uint16_t)((uint32_t)key & fmask);
+            if (!bv.tpt) bv.tfine[pos] = (uint16_t)((uint32_t)key & fmask);
         }
     }
     __syncthreads();
@@ -484,9 +484,26 @@ __device__ __forceinline__ void fine_body(const BuildView& bv, int b, int* s_mem
     int* s_cur = s_mem + tile;
     const int bs = bv.bstart[b];
     const int tot = bv.btotal[b];
+    // pixel within the bin of binned point i: stored by the bin pass, or, for
+    // binned (x, y, z, index) records, recomputed from the coordinates by the
+    // same projection (bit-identical to the bin pass's: the same operations
+    // in the same order), which saves the bin pass a second scattered write
+    // per point
+    ProjParams pp;
+    if (bv.tpt) load_proj(bv.w2c, bv.K, bv.H, bv.W, pp);
+    const uint32_t fmask = (uint32_t)tile - 1u;
+    auto fine_of = [&](int i, float4& q) -> uint32_t {
+        if (!bv.tpt) return bv.tfine[i];
+        q = bv.tpt[i];
+        float depth;
+        return (uint32_t)project_key(pp, q.x, q.y, q.z, depth) & fmask;
+    };
     for (int f = threadIdx.x; f < tile; f += kBuildBlock) s_cnt[f] = 0;
     __syncthreads();
-    for (int i = bs + threadIdx.x; i < bs + tot; i += kBuildBlock) atomicAdd(&s_cnt[bv.tfine[i]], 1);
+    for (int i = bs + threadIdx.x; i < bs + tot; i += kBuildBlock) {
+        float4 q;
+        atomicAdd(&s_cnt[fine_of(i, q)], 1);
+    }
     __syncthreads();
     for (int f = threadIdx.x; f < tile; f += kBuildBlock) s_cur[f] = s_cnt[f];
     __syncthreads();
@@ -505,11 +522,11 @@ __device__ __forceinline__ void fine_body(const BuildView& bv, int b, int* s_mem
     for (int r0 = bs; r0 < bs + tot; r0 += kBuildBlock) {
         const int i = r0 + threadIdx.x;
         const bool active = i < bs + tot;
-        const uint32_t f = active ? bv.tfine[i] : 0u;
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t f = active ? fine_of(i, q) : 0u;
         const int pos = ordered_append(s_cur, f, ts, active);
         if (active) {
             if (bv.tpt) {
-                const float4 q = bv.tpt[i];
                 bv.out_idx[bs + pos] = __float_as_int(q.w);
                 bv.out_pts[bs + pos] = q;
             } else {
