@@ -17,13 +17,8 @@ namespace {
 
 constexpr int kBlock = kSampleBlock;
 
-// At least three waves per SIMD: the feature layout with four-word lane sets
-// needs ~180 registers, which would allow only two, and its large windows
-// wait on candidate loads from global memory (20M points at k = 15: 85 ms per
-// 1080p view at two waves, 73.6 ms at three with ~16 registers spilled);
-// every other specialisation already fits.
 template <int MODE, int KMAX, int SMAX, int LAY, bool PR, int NW>
-__global__ __launch_bounds__(kBlock, 3) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
+__global__ __launch_bounds__(kBlock) void sample_tile_kernel(SampleOpts o, SampleOut out, ViewArgs view, TilePlan tp,
                                                              int tiles_x) {
     extern __shared__ float4 s_dyn[];
     __shared__ int s_total;
