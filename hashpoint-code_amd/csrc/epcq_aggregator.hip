@@ -99,9 +99,11 @@ __device__ __forceinline__ void strip_layer(uint16_t* io, const uint16_t* __rest
     const int row = lane & 31;
     const int kh = (lane >> 5) * 8;
     constexpr int K = 16 * KS;
+    // every accumulator of tile t in this lane is output column 32 t + row:
+    // it starts at that column's bias
     f32x16 acc[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+    for (int t = 0; t < NT; ++t) acc[t] = (f32x16)(bias[t * 32 + row]);
     const uint16_t* a_row = io + row * kLdsStride + kh;
     const uint16_t* w_row = W + (size_t)row * K + kh;
     // the weight operands of steps ks + 1 and ks + 2 load while step ks
@@ -131,11 +133,10 @@ __device__ __forceinline__ void strip_layer(uint16_t* io, const uint16_t* __rest
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int c = t * 32 + row;
-        const float bc = bias[c];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int rr = (r >> 2) * 8 + r0 + (r & 3);
-            float v = acc[t][r] + bc;
+            float v = acc[t][r];
             if (LEAKY) v = lrelu(v);
             io[rr * kLdsStride + c] = to_bf16(v);
         }
@@ -162,9 +163,9 @@ __device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const u
     uint4 c0, c1;
     __builtin_memcpy(&c0, wsrc, 16);
     __builtin_memcpy(&c1, wsrc + 16, 16);
-    f32x16 acc[NT];
+    f32x16 acc[NT];  // seeded with the bias of the lane's output column, as in strip_layer
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+    for (int t = 0; t < NT; ++t) acc[t] = (f32x16)(bias[t * 32 + row]);
     const uint16_t* a_row = io + row * kLdsStride + kh;
 #pragma unroll 1
     for (int ks = 0; ks < KS; ++ks) {
@@ -187,11 +188,10 @@ __device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const u
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int c = t * 32 + row;
-        const float bc = bias[c];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int rr = (r >> 2) * 8 + r0 + (r & 3);
-            io[rr * kLdsStride + c] = to_bf16(lrelu(acc[t][r] + bc));
+            io[rr * kLdsStride + c] = to_bf16(lrelu(acc[t][r]));
         }
     }
 }
