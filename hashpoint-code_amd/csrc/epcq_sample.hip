@@ -46,31 +46,6 @@ float squared_threshold(float r) {
     return s;
 }
 
-// Dynamic LDS one sampling block may take: the current device's per-block
-// maximum (160 KB on gfx950), less a reserve for the kernels' static arrays
-// (the deferred-ray list and a few counters, about 1 KB).
-size_t tile_lds_limit() {
-    constexpr size_t kStaticReserve = 2048;
-    constexpr int kMaxDevices = 64;
-    static std::mutex mu;
-    static size_t limit[kMaxDevices] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
-        (void)hipGetLastError();
-        return 64 * 1024 - kStaticReserve;
-    }
-    std::lock_guard<std::mutex> lk(mu);
-    if (!limit[dev]) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
-            (void)hipGetLastError();
-            v = 64 * 1024;
-        }
-        limit[dev] = (size_t)v > 2 * kStaticReserve ? (size_t)v - kStaticReserve : (size_t)v / 2;
-    }
-    return limit[dev];
-}
-
 TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags, bool point_radius) {
     TilePlan p{};
     p.tw = tile_w;
@@ -81,7 +56,8 @@ TilePlan plan_tiles(int k, int tile_w, int tile_h, int cap, int flags, bool poin
     // per point: float4 + radius; per region pixel: cone_a, hash, cone_b; per row: lo, base
     const size_t fixed =
         align_up((size_t)p.rw_max * p.rh_max * (16 + 8 + 8) + (size_t)(2 * p.rh_max + 1) * 4, 16);
-    const size_t limit = tile_lds_limit();
+    // 64 KB: a larger stage leaves one block per CU, slower on every workload measured
+    const size_t limit = 64 * 1024;
     size_t c = (size_t)std::max(cap, 0);
     const size_t per = point_radius ? 20 : 16;
     if (fixed + c * per > limit) c = fixed < limit ? (limit - fixed) / per : 0;

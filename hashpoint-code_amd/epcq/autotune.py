@@ -18,7 +18,7 @@ import torch
 from . import csrc as _C
 
 SHAPES = [(16, 16), (8, 8), (16, 8), (32, 8), (32, 4), (64, 2)]
-CAPS = [1024, 512, 2048, 3000, 6000]
+CAPS = [1024, 512, 2048, 3000]
 FLAGS = [0, _C.TUNE_XCD_ORDER]
 HEAVY = [_C.HEAVY_DEFAULT, 64, 256, 0]
 DEFAULT = {"tile_w": 32, "tile_h": 8, "tile_cap": 1024, "flags": 0, "heavy": _C.HEAVY_DEFAULT}

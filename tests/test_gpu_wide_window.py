@@ -58,10 +58,10 @@ class launch_shape:
 @pytest.mark.parametrize("cap", [6000, 1024])
 @pytest.mark.parametrize("mode", ["knn", "compat"])
 def test_wide_window_staging_capacity(dev, cap, mode):
-    """k = 15 with per-point radii through 16x16 tiles: a 6000-point stage
-    takes about 150 KB of LDS per block (more than 64 KB needs gfx950's
-    160 KB), a 1024-point one overflows and reads candidates from global
-    memory; both against the oracle."""
+    """k = 15 with per-point radii through 16x16 tiles: a 6000-point request
+    is clamped to what fits 64 KB of LDS with the tile's hash entries and
+    cones (about 1800 points), a 1024-point one overflows more often and
+    reads candidates from global memory; both against the oracle."""
     pts = synth.box_scene(250_000, n_boxes=20, seed=11)
     radius = synth.log_uniform_radius(len(pts), seed=12, lo=2e-3, hi=2e-2)
     cam = synth.look_at_camera([2.4, 2.0, 1.8], 96, 80, synth.nerf_focal(96) * 2.0)
