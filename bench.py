@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "serving", "box", "sphere", "large", "aggregator"])
+                    choices=["batched", "serving", "box", "sphere", "large", "aggregator", "render", "rays"])
     ap.add_argument("--mode", default=None, choices=["compat", "knn"],
                     help="neighbour mode (default: compat, knn for the large workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -691,6 +691,160 @@ def bench_aggregator(args):
     }
 
 
+def bench_render(args):
+    """Volume-render epilogue (clamp_t fused into render, and render's
+    backward) on the rays of an 800x800 image with S = 16 samples each:
+    memory-bound per-ray scans.  Algorithmic bytes per ray: forward reads t,
+    sigma (4S each) and rgb (12S) and writes the colour (12); backward also
+    reads the colour gradient (12) and writes the sigma and rgb gradients
+    (4S + 12S)."""
+    import numpy as np
+    import torch
+
+    import epcq.csrc as C
+    from oracle import oracle
+
+    dev = torch.device("cuda:0")
+    R, S = 800 * 800, 16
+    rng = np.random.default_rng(5)
+    gaps = rng.uniform(0.0, 0.012, size=(R, S))
+    t = (rng.uniform(2.0, 6.0, size=(R, 1)) + np.cumsum(gaps, axis=1)).astype(np.float32)
+    sigma = rng.exponential(20.0, size=(R, S)).astype(np.float32)
+    rgb = rng.uniform(0, 1, size=(R, S, 3)).astype(np.float32)
+    g = rng.normal(size=(R, 3)).astype(np.float32)
+    td, sd, cd, gd = (torch.from_numpy(x).to(dev) for x in (t, sigma, rgb, g))
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, n):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+        for e0, e1 in evs:
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+
+    fwd = timed(lambda: C.render_rays(sd, cd, white_bg=True, t_val=td), args.steps)
+    bwd = timed(lambda: C.render_rays_backward(sd, cd, gd, white_bg=True, t_val=td), args.steps)
+    fb = R * (20 * S + 12)
+    bb = R * (20 * S + 12 + 16 * S)
+    # the reference's torch formulation (epcq.py:508-530) on this GPU
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_oracle_render import torch_reference
+
+    ref = timed(lambda: torch_reference(td, sd, cd, True), 3)
+    n_cpu = 100_000
+    oracle.render(sigma[:n_cpu], rgb[:n_cpu], True, t=t[:n_cpu])
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.render(sigma[:n_cpu], rgb[:n_cpu], True, t=t[:n_cpu])
+        cpu.append(time.perf_counter() - t0)
+    ach = fb / (fwd * 1e-3) / 1e9
+    log(f"render: forward {fwd:.3f} ms ({ach:.0f} GB/s), backward {bwd:.3f} ms "
+        f"({bb / (bwd * 1e-3) / 1e9:.0f} GB/s), torch fp32 {ref:.3f} ms")
+    return {
+        "metric": "rays/s volume-render epilogue (clamp_t + render forward)", "value": R / (fwd * 1e-3),
+        "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "render-800x800-S16", "rays": R, "samples_per_ray": S, "white_bg": True},
+        "backward": {"ms": bwd, "rays_per_s": R / (bwd * 1e-3), "achieved_GBs": bb / (bwd * 1e-3) / 1e9,
+                     "bytes_per_launch": bb},
+        "roofline": {"bound": "hbm", "kernel": "render_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": fb,
+                     "bytes_model": "per ray 4S (t) + 4S (sigma) + 12S (rgb) + 12 (colour)"},
+        "torch_fp32_gpu": {"ms_per_step": ref, "speedup": ref / fwd},
+        "cpu_baseline": {"value": n_cpu / float(np.median(cpu)), "unit": "rays/s", "cores": 1, "kind": "port",
+                         "cpu_model": cpu_model(), "sample": f"{n_cpu} rays through the oracle's render, "
+                                                              "median of 3 after a warm-up"},
+    }
+
+
+def bench_rays(args):
+    """Training ray subset (opt/render_image.py:82-88): a batch of 64 x 64
+    random rays of an 800x800 view of the 1M-point box scene sampled in the
+    dense drop-in layout (S = 16, Kc = 8), by render_image_hashtable_topk
+    with training_ray_index, against the reference's way: sample the whole
+    image, then index the outputs with the batch.  Each step includes the
+    hash-table build of the view (pts2plane), as a training step does."""
+    import numpy as np
+    import torch
+
+    from epcq import synth
+    from epcq.epcq import NeuralPoints, RenderOptions
+    from oracle import oracle
+
+    dev = torch.device("cuda:0")
+    pts, cams, kw = synth.workload_box_scene()
+    cam = cams[0]
+    opt = RenderOptions(**dict(kw, num_point_cloud_per_sp=8))
+    npts = NeuralPoints(pc_type="pc", points=torch.from_numpy(pts).to(dev))
+    cs = camera_specs([cam], dev)[0]
+    rng = np.random.default_rng(6)
+    B = 64 * 64
+    uv = np.stack([rng.integers(0, cam.width, B), rng.integers(0, cam.height, B)], axis=1).astype(np.int64)
+    tri = torch.from_numpy(uv).to(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def subset():
+        idx, depth, table = npts.pts2plane(cs, opt)
+        return npts.render_image_hashtable_topk(cs, opt, idx, depth, table, training_ray_index=tri)
+
+    def full_then_index():
+        idx, depth, table = npts.pts2plane(cs, opt)
+        sp, pc, st, m = npts.render_image_hashtable_topk(cs, opt, idx, depth, table)
+        v, u = tri[:, 1], tri[:, 0]
+        return sp[v, u], pc[v, u], st[v, u], m[v, u]
+
+    def timed(fn, n):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+        for e0, e1 in evs:
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+
+    a = subset()
+    b = full_then_index()
+    torch.cuda.synchronize()
+    for x, y in zip(a, b):
+        assert torch.equal(x, y), "subset sampling differs from full-image sampling + indexing"
+    ms_sub = timed(subset, args.steps)
+    ms_full = timed(full_then_index, args.steps)
+    # CPU: the oracle's build and its sampling of the same ray list
+    oi, od, oh = oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width)
+    rays = (uv[:, 1] * cam.width + uv[:, 0]).astype(np.int32)
+    ocam, oopt = oracle.camera_from(cam), oracle.options_from(opt)
+    cpu = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.pts2plane(pts, cam.w2c, cam.K, cam.height, cam.width)
+        oracle.quick_sampling(pts, ocam, oopt, oi, oh, mode=oracle.MODE_COMPAT, rays=rays)
+        cpu.append(time.perf_counter() - t0)
+    log(f"training rays: {ms_sub:.3f} ms per batch of {B} (build + subset sampling), "
+        f"full image + indexing {ms_full:.3f} ms")
+    return {
+        "metric": "rays/s training ray batch (build + sampling of the batch's rays)", "value": B / (ms_sub * 1e-3),
+        "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_sub,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "box20-1M-800x800-batch4096", "points": len(pts), "rays": B, "S": opt.num_sample_points_per_ray,
+                   "Kc": opt.num_point_cloud_per_sp, "layout": "dense drop-in"},
+        "full_image_then_index": {"ms_per_step": ms_full, "speedup": ms_full / ms_sub},
+        "roofline": None,
+        "cpu_baseline": {"value": B / float(np.median(cpu)), "unit": "rays/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                         "kind": "port", "cpu_model": cpu_model(),
+                         "sample": "the oracle's build of the view and its sampling of the same 4096 rays, "
+                                   "median of 3"},
+    }
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of `mib` MiB, five times: the reference
     point that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -720,6 +874,12 @@ def main():
         return
     if args.workload == "aggregator":
         print(json.dumps(bench_aggregator(args)), flush=True)
+        return
+    if args.workload == "render":
+        print(json.dumps(bench_render(args)), flush=True)
+        return
+    if args.workload == "rays":
+        print(json.dumps(bench_rays(args)), flush=True)
         return
     wl = load_workload(args.workload)
     if args.workload == "batched":

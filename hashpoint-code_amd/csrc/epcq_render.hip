@@ -36,9 +36,45 @@ __device__ __forceinline__ float clamp_delta(const float* __restrict__ t, int S,
     return d * (1.0f - m) + m * vsize;
 }
 
-__device__ __forceinline__ float row_delta(const float* __restrict__ t, const float* __restrict__ delta, int S,
-                                           int k, float vsize) {
-    return delta ? delta[k] : clamp_delta(t, S, k, vsize);
+
+// clamp_t of a sample whose depth tk and next depth tk1 are in registers
+// (last: the row's last sample, whose interval is vsize); the same
+// operations as clamp_delta.
+__device__ __forceinline__ float clamp_delta_v(float tk, float tk1, bool last, float vsize) {
+    const float d = last ? vsize : tk1 - tk;
+    const float m = (d < 1e-8f || d > 2.0f * vsize) ? 1.0f : 0.0f;
+    return d * (1.0f - m) + m * vsize;
+}
+
+// Samples [k0, k0 + n) of a row (n <= kChunk) in registers: their
+// intervals, densities and colours.  All loads of the chunk are issued before
+// any is used, so a lane waits for memory once per chunk, not once per sample.
+constexpr int kChunk = 16;
+struct RowChunk {
+    float d[kChunk], s[kChunk], c[3 * kChunk];
+};
+
+__device__ __forceinline__ void load_chunk(const float* __restrict__ tr, const float* __restrict__ dr,
+                                           const float* __restrict__ sr, const float* __restrict__ cr, int S,
+                                           int k0, int n, float vsize, RowChunk& ch) {
+    float tv[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+        if (j < n) {
+            ch.s[j] = sr[k0 + j];
+            ch.c[3 * j] = cr[3 * (k0 + j)];
+            ch.c[3 * j + 1] = cr[3 * (k0 + j) + 1];
+            ch.c[3 * j + 2] = cr[3 * (k0 + j) + 2];
+            if (dr) ch.d[j] = dr[k0 + j];
+            else tv[j] = tr[k0 + j];
+        }
+    }
+    if (!dr) {
+        const float tnext = k0 + n < S ? tr[k0 + n] : 0.0f;  // the depth after the chunk
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j)
+            if (j < n) ch.d[j] = clamp_delta_v(tv[j], j + 1 < n ? tv[min(j + 1, kChunk - 1)] : tnext, k0 + j + 1 >= S, vsize);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void clamp_t_kernel(const float* __restrict__ t, int64_t R, int S, float vsize,
@@ -63,15 +99,22 @@ __global__ __launch_bounds__(kBlock) void render_kernel(const float* __restrict_
     float T = 1.0f, Tlast = 1.0f;
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
     const int live = (t && S == 1) ? 0 : S;
-    for (int k = 0; k < live; ++k) {
-        const float dk = row_delta(tr, dr, S, k, vsize);
-        const float op = 1.0f - expf_cw(-sr[k] * dk);
-        const float w = op * T;
-        c0 = c0 + w * cr[3 * k];
-        c1 = c1 + w * cr[3 * k + 1];
-        c2 = c2 + w * cr[3 * k + 2];
-        Tlast = T;
-        T = T * ((1.0f - op) + 1e-10f);
+    for (int k0 = 0; k0 < live; k0 += kChunk) {
+        const int n = min(kChunk, live - k0);
+        RowChunk ch;
+        load_chunk(tr, dr, sr, cr, S, k0, n, vsize, ch);
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j < n) {
+                const float op = 1.0f - expf_cw(-ch.s[j] * ch.d[j]);
+                const float w = op * T;
+                c0 = c0 + w * ch.c[3 * j];
+                c1 = c1 + w * ch.c[3 * j + 1];
+                c2 = c2 + w * ch.c[3 * j + 2];
+                Tlast = T;
+                T = T * ((1.0f - op) + 1e-10f);
+            }
+        }
     }
     out[3 * r] = c0 + Tlast * bg;
     out[3 * r + 1] = c1 + Tlast * bg;
@@ -96,32 +139,48 @@ __global__ __launch_bounds__(kBlock) void render_backward_kernel(
     }
     // pass 1: the total of op_i T_i (g . c_i) and T_{S-1}
     float T = 1.0f, Tlast = 1.0f, total = 0.0f;
-    for (int k = 0; k < S; ++k) {
-        const float dk = row_delta(tr, dr, S, k, vsize);
-        const float op = 1.0f - expf_cw(-sr[k] * dk);
-        const float gc = dot3(g0, g1, g2, cr[3 * k], cr[3 * k + 1], cr[3 * k + 2]);
-        total = total + op * T * gc;
-        Tlast = T;
-        T = T * ((1.0f - op) + 1e-10f);
+    for (int k0 = 0; k0 < S; k0 += kChunk) {
+        const int n = min(kChunk, S - k0);
+        RowChunk ch;
+        load_chunk(tr, dr, sr, cr, S, k0, n, vsize, ch);
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j < n) {
+                const float op = 1.0f - expf_cw(-ch.s[j] * ch.d[j]);
+                const float gc = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
+                total = total + op * T * gc;
+                Tlast = T;
+                T = T * ((1.0f - op) + 1e-10f);
+            }
+        }
     }
     const float gbg = (g0 + g1 + g2) * bg;
     // pass 2: per-sample gradients with the suffix sums
     T = 1.0f;
     float prefix = 0.0f;
-    for (int k = 0; k < S; ++k) {
-        const float dk = row_delta(tr, dr, S, k, vsize);
-        const float e = expf_cw(-sr[k] * dk);
-        const float op = 1.0f - e;
-        const float a = (1.0f - op) + 1e-10f;
-        const float gc = dot3(g0, g1, g2, cr[3 * k], cr[3 * k + 1], cr[3 * k + 2]);
-        const float wk = op * T;
-        prefix = prefix + wk * gc;
-        const float after = (total - prefix) + (k + 1 < S ? Tlast * gbg : 0.0f);
-        grad_sigma[r * S + k] = dk * e * (T * gc - after / a);
-        grad_rgb[(r * S + k) * 3] = wk * g0;
-        grad_rgb[(r * S + k) * 3 + 1] = wk * g1;
-        grad_rgb[(r * S + k) * 3 + 2] = wk * g2;
-        T = T * a;
+    for (int k0 = 0; k0 < S; k0 += kChunk) {
+        const int n = min(kChunk, S - k0);
+        RowChunk ch;
+        load_chunk(tr, dr, sr, cr, S, k0, n, vsize, ch);
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j < n) {
+                const int k = k0 + j;
+                const float dk = ch.d[j];
+                const float e = expf_cw(-ch.s[j] * dk);
+                const float op = 1.0f - e;
+                const float a = (1.0f - op) + 1e-10f;
+                const float gc = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
+                const float wk = op * T;
+                prefix = prefix + wk * gc;
+                const float after = (total - prefix) + (k + 1 < S ? Tlast * gbg : 0.0f);
+                grad_sigma[r * S + k] = dk * e * (T * gc - after / a);
+                grad_rgb[(r * S + k) * 3] = wk * g0;
+                grad_rgb[(r * S + k) * 3 + 1] = wk * g1;
+                grad_rgb[(r * S + k) * 3 + 2] = wk * g2;
+                T = T * a;
+            }
+        }
     }
 }
 
