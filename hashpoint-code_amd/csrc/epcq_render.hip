@@ -54,8 +54,7 @@ struct RowChunk {
     float d[kChunk], s[kChunk], c[3 * kChunk];
 };
 
-__device__ __forceinline__ void load_chunk(const float* __restrict__ tr, const float* __restrict__ dr,
-                                           const float* __restrict__ sr, const float* __restrict__ cr, int S,
+__device__ __forceinline__ void load_chunk(const float* tr, const float* dr, const float* sr, const float* cr, int S,
                                            int k0, int n, float vsize, RowChunk& ch) {
     float tv[kChunk];
 #pragma unroll
@@ -86,19 +85,13 @@ __global__ __launch_bounds__(kBlock) void clamp_t_kernel(const float* __restrict
     delta[i] = clamp_delta(t + r * S, S, k, vsize);
 }
 
-__global__ __launch_bounds__(kBlock) void render_kernel(const float* __restrict__ t, const float* __restrict__ delta,
-                                                        float vsize, const float* __restrict__ sigma,
-                                                        const float* __restrict__ rgb, int64_t R, int S, float bg,
-                                                        float* __restrict__ out) {
-    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (r >= R) return;
-    const float* tr = t ? t + r * S : nullptr;
-    const float* dr = delta ? delta + r * S : nullptr;
-    const float* sr = sigma + r * S;
-    const float* cr = rgb + r * S * 3;
+// The forward walk of one row whose samples sit at tr (or dr), sr and cr:
+// global memory or LDS rows.  Returns the ray's colour in o0..o2.
+__device__ __forceinline__ void render_row(const float* tr, const float* dr, const float* sr, const float* cr, int S,
+                                           float vsize, float bg, float& o0, float& o1, float& o2) {
     float T = 1.0f, Tlast = 1.0f;
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    const int live = (t && S == 1) ? 0 : S;
+    const int live = (tr && S == 1) ? 0 : S;
     for (int k0 = 0; k0 < live; k0 += kChunk) {
         const int n = min(kChunk, live - k0);
         RowChunk ch;
@@ -116,25 +109,21 @@ __global__ __launch_bounds__(kBlock) void render_kernel(const float* __restrict_
             }
         }
     }
-    out[3 * r] = c0 + Tlast * bg;
-    out[3 * r + 1] = c1 + Tlast * bg;
-    out[3 * r + 2] = c2 + Tlast * bg;
+    o0 = c0 + Tlast * bg;
+    o1 = c1 + Tlast * bg;
+    o2 = c2 + Tlast * bg;
 }
 
-__global__ __launch_bounds__(kBlock) void render_backward_kernel(
-    const float* __restrict__ t, const float* __restrict__ delta, float vsize, const float* __restrict__ sigma,
-    const float* __restrict__ rgb, const float* __restrict__ grad_out, int64_t R, int S, float bg,
-    float* __restrict__ grad_sigma, float* __restrict__ grad_rgb) {
-    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (r >= R) return;
-    const float* tr = t ? t + r * S : nullptr;
-    const float* dr = delta ? delta + r * S : nullptr;
-    const float* sr = sigma + r * S;
-    const float* cr = rgb + r * S * 3;
-    const float g0 = grad_out[3 * r], g1 = grad_out[3 * r + 1], g2 = grad_out[3 * r + 2];
-    if (t && S == 1) {  // no sample contributes (see above)
-        grad_sigma[r] = 0.0f;
-        grad_rgb[3 * r] = grad_rgb[3 * r + 1] = grad_rgb[3 * r + 2] = 0.0f;
+// The backward walk of one row for the colour gradient g0..g2: writes
+// dL/dsigma to gs[k] and dL/dc to gc[3k..3k+2].  gs and gc may be the rows sr
+// and cr themselves: sample k's gradients are written after its inputs are
+// read, and no later sample reads them.
+__device__ __forceinline__ void render_row_backward(const float* tr, const float* dr, const float* sr, const float* cr,
+                                                    int S, float vsize, float bg, float g0, float g1, float g2,
+                                                    float* gs, float* gc) {
+    if (tr && S == 1) {  // no sample contributes (see above)
+        gs[0] = 0.0f;
+        gc[0] = gc[1] = gc[2] = 0.0f;
         return;
     }
     // pass 1: the total of op_i T_i (g . c_i) and T_{S-1}
@@ -170,18 +159,178 @@ __global__ __launch_bounds__(kBlock) void render_backward_kernel(
                 const float e = expf_cw(-ch.s[j] * dk);
                 const float op = 1.0f - e;
                 const float a = (1.0f - op) + 1e-10f;
-                const float gc = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
+                const float gck = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
                 const float wk = op * T;
-                prefix = prefix + wk * gc;
+                prefix = prefix + wk * gck;
                 const float after = (total - prefix) + (k + 1 < S ? Tlast * gbg : 0.0f);
-                grad_sigma[r * S + k] = dk * e * (T * gc - after / a);
-                grad_rgb[(r * S + k) * 3] = wk * g0;
-                grad_rgb[(r * S + k) * 3 + 1] = wk * g1;
-                grad_rgb[(r * S + k) * 3 + 2] = wk * g2;
+                gs[k] = dk * e * (T * gck - after / a);
+                gc[3 * k] = wk * g0;
+                gc[3 * k + 1] = wk * g1;
+                gc[3 * k + 2] = wk * g2;
                 T = T * a;
             }
         }
     }
+}
+
+// Direct forms: each lane reads its own row from global memory (any S).
+__global__ __launch_bounds__(kBlock) void render_kernel(const float* __restrict__ t, const float* __restrict__ delta,
+                                                        float vsize, const float* __restrict__ sigma,
+                                                        const float* __restrict__ rgb, int64_t R, int S, float bg,
+                                                        float* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= R) return;
+    float o0, o1, o2;
+    render_row(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S, rgb + r * S * 3, S, vsize, bg,
+               o0, o1, o2);
+    out[3 * r] = o0;
+    out[3 * r + 1] = o1;
+    out[3 * r + 2] = o2;
+}
+
+__global__ __launch_bounds__(kBlock) void render_backward_kernel(
+    const float* __restrict__ t, const float* __restrict__ delta, float vsize, const float* __restrict__ sigma,
+    const float* __restrict__ rgb, const float* __restrict__ grad_out, int64_t R, int S, float bg,
+    float* __restrict__ grad_sigma, float* __restrict__ grad_rgb) {
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= R) return;
+    render_row_backward(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S, rgb + r * S * 3, S,
+                        vsize, bg, grad_out[3 * r], grad_out[3 * r + 1], grad_out[3 * r + 2], grad_sigma + r * S,
+                        grad_rgb + r * S * 3);
+}
+
+// Staged forms, for rows that fit: a one-wave block takes 64 consecutive rays,
+// copies their rows of t (or delta), sigma and rgb into LDS with coalesced
+// loads (the wave's lanes read consecutive floats), walks each ray from LDS,
+// and (backward) writes the gradients back the same way.  The LDS rows have an
+// odd stride, so the 64 lanes reading sample k of their rows hit 64 different
+// banks.  The arithmetic is the direct form's, operation for operation.
+constexpr int kStageRays = 64;
+constexpr int kStageUnroll = 16;                  // loads of t and of sigma per lane and round
+constexpr size_t kStageLdsMax = 64 * 1024;        // beyond this, the direct form
+
+__host__ __device__ constexpr int odd_stride(int w) { return w | 1; }
+
+__host__ __device__ constexpr size_t stage_lds_bytes(int S) {
+    return (size_t)kStageRays * sizeof(float) * (2 * odd_stride(S) + odd_stride(3 * S));
+}
+
+// Advances a lane's (row, column) in rows of width w by kStageRays flat
+// elements: (q, m) = (kStageRays / w, kStageRays % w), with a carry.
+__device__ __forceinline__ void step_rc(int& row, int& col, int q, int m, int w) {
+    col += m;
+    row += q;
+    if (col >= w) col -= w, ++row;
+}
+
+// Copies the block's nr rows of t (or delta) and sigma (width S) and of rgb
+// (width 3S), contiguous in global memory from a, b and c, into LDS rows of
+// stride ws and wc.  Lane l loads the flat elements l, l + 64, ...; a round
+// issues kStageUnroll loads of a and b and 3 kStageUnroll of c before it
+// stores any, so a wave waits for memory once per round (once in all for
+// S <= 16).
+__device__ __forceinline__ void stage_rows(const float* __restrict__ a, const float* __restrict__ b,
+                                           const float* __restrict__ c, float* sa, float* sb, float* sc, int nr,
+                                           int S, int ws, int wc) {
+    constexpr int U = kStageUnroll;
+    const int lane = threadIdx.x, n1 = nr * S, n3 = 3 * n1, w3 = 3 * S;
+    const int q1 = kStageRays / S, m1 = kStageRays - q1 * S;
+    const int q3 = kStageRays / w3, m3 = kStageRays - q3 * w3;
+    int r1 = lane / S, c1 = lane - r1 * S;
+    int r3 = lane / w3, c3 = lane - r3 * w3;
+    // round k covers elements [k 64 U, (k + 1) 64 U) of a and b and three
+    // times that span of c: the rounds of a and b cover c as well
+    for (int base = 0; base < n1; base += kStageRays * U) {
+        const int i0 = base + lane, j0 = 3 * base + lane;
+        float va[U], vb[U], vc[3 * U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + kStageRays * u < n1) {
+                va[u] = a[i0 + kStageRays * u];
+                vb[u] = b[i0 + kStageRays * u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 3 * U; ++u)
+            if (j0 + kStageRays * u < n3) vc[u] = c[j0 + kStageRays * u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + kStageRays * u < n1) {
+                sa[r1 * ws + c1] = va[u];
+                sb[r1 * ws + c1] = vb[u];
+                step_rc(r1, c1, q1, m1, S);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 3 * U; ++u) {
+            if (j0 + kStageRays * u < n3) {
+                sc[r3 * wc + c3] = vc[u];
+                step_rc(r3, c3, q3, m3, w3);
+            }
+        }
+    }
+}
+
+// Writes the nr LDS rows of stride ws back to the contiguous rows of width w
+// at g, the wave's lanes storing consecutive floats.
+__device__ __forceinline__ void unstage_rows(float* __restrict__ g, const float* s, int nr, int w, int ws) {
+    const int lane = threadIdx.x, total = nr * w;
+    const int q = kStageRays / w, m = kStageRays - q * w;
+    int row = lane / w, col = lane - row * w;
+    for (int i = lane; i < total; i += kStageRays) {
+        g[i] = s[row * ws + col];
+        step_rc(row, col, q, m, w);
+    }
+}
+
+__global__ __launch_bounds__(kStageRays) void render_staged_kernel(const float* __restrict__ t,
+                                                                   const float* __restrict__ delta, float vsize,
+                                                                   const float* __restrict__ sigma,
+                                                                   const float* __restrict__ rgb, int64_t R, int S,
+                                                                   float bg, float* __restrict__ out) {
+    extern __shared__ float s_rows[];
+    const int ws = odd_stride(S), wc = odd_stride(3 * S);
+    float* st = s_rows;
+    float* ss = st + kStageRays * ws;
+    float* sc = ss + kStageRays * ws;
+    const int64_t r0 = (int64_t)blockIdx.x * kStageRays;
+    const int nr = (int)min<int64_t>(kStageRays, R - r0);
+    stage_rows((t ? t : delta) + r0 * S, sigma + r0 * S, rgb + r0 * 3 * S, st, ss, sc, nr, S, ws, wc);
+    __syncthreads();
+    const int l = threadIdx.x;
+    if (l >= nr) return;
+    float o0, o1, o2;
+    render_row(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws, sc + l * wc, S, vsize, bg, o0, o1,
+               o2);
+    const int64_t r = r0 + l;
+    out[3 * r] = o0;
+    out[3 * r + 1] = o1;
+    out[3 * r + 2] = o2;
+}
+
+__global__ __launch_bounds__(kStageRays) void render_backward_staged_kernel(
+    const float* __restrict__ t, const float* __restrict__ delta, float vsize, const float* __restrict__ sigma,
+    const float* __restrict__ rgb, const float* __restrict__ grad_out, int64_t R, int S, float bg,
+    float* __restrict__ grad_sigma, float* __restrict__ grad_rgb) {
+    extern __shared__ float s_rows[];
+    const int ws = odd_stride(S), wc = odd_stride(3 * S);
+    float* st = s_rows;
+    float* ss = st + kStageRays * ws;
+    float* sc = ss + kStageRays * ws;
+    const int64_t r0 = (int64_t)blockIdx.x * kStageRays;
+    const int nr = (int)min<int64_t>(kStageRays, R - r0);
+    stage_rows((t ? t : delta) + r0 * S, sigma + r0 * S, rgb + r0 * 3 * S, st, ss, sc, nr, S, ws, wc);
+    __syncthreads();
+    const int l = threadIdx.x;
+    if (l < nr) {
+        const int64_t r = r0 + l;
+        // the gradients replace the row's sigma and rgb in LDS
+        render_row_backward(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws, sc + l * wc, S, vsize,
+                            bg, grad_out[3 * r], grad_out[3 * r + 1], grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
+    }
+    __syncthreads();
+    unstage_rows(grad_sigma + r0 * S, ss, nr, S, ws);
+    unstage_rows(grad_rgb + r0 * 3 * S, sc, nr, 3 * S, wc);
 }
 
 int check_render(const float* t, const float* delta, int64_t R, int32_t S, const char* who) {
@@ -208,8 +357,12 @@ extern "C" int epcq_render_rays(const float* t, const float* delta, float vsize,
                                 int64_t R, int32_t S, int32_t white_bg, float* out_rgb, void* stream) {
     if (int e = check_render(t, delta, R, S, "render")) return e;
     if (R == 0) return EPCQ_OK;
-    render_kernel<<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, as_stream(stream)>>>(
-        t, delta, vsize, sigma, rgb, R, S, white_bg ? 1.0f : 0.0f, out_rgb);
+    if (S > 0 && stage_lds_bytes(S) <= kStageLdsMax)
+        render_staged_kernel<<<dim3((unsigned)ceil_div(R, kStageRays)), dim3(kStageRays), stage_lds_bytes(S),
+                               as_stream(stream)>>>(t, delta, vsize, sigma, rgb, R, S, white_bg ? 1.0f : 0.0f, out_rgb);
+    else
+        render_kernel<<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, as_stream(stream)>>>(
+            t, delta, vsize, sigma, rgb, R, S, white_bg ? 1.0f : 0.0f, out_rgb);
     return check_launch("render");
 }
 
@@ -218,7 +371,12 @@ extern "C" int epcq_render_rays_backward(const float* t, const float* delta, flo
                                          int32_t white_bg, float* grad_sigma, float* grad_rgb, void* stream) {
     if (int e = check_render(t, delta, R, S, "render_backward")) return e;
     if (R == 0 || S == 0) return EPCQ_OK;
-    render_backward_kernel<<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, as_stream(stream)>>>(
-        t, delta, vsize, sigma, rgb, grad_rgb_out, R, S, white_bg ? 1.0f : 0.0f, grad_sigma, grad_rgb);
+    if (stage_lds_bytes(S) <= kStageLdsMax)
+        render_backward_staged_kernel<<<dim3((unsigned)ceil_div(R, kStageRays)), dim3(kStageRays),
+                                        stage_lds_bytes(S), as_stream(stream)>>>(
+            t, delta, vsize, sigma, rgb, grad_rgb_out, R, S, white_bg ? 1.0f : 0.0f, grad_sigma, grad_rgb);
+    else
+        render_backward_kernel<<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, as_stream(stream)>>>(
+            t, delta, vsize, sigma, rgb, grad_rgb_out, R, S, white_bg ? 1.0f : 0.0f, grad_sigma, grad_rgb);
     return check_launch("render_backward");
 }

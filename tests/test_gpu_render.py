@@ -16,7 +16,12 @@ from test_oracle_render import random_rays, torch_reference
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("R,S", [(1, 1), (1000, 16), (4097, 7), (300, 64)])
+# S up to 51 takes the LDS-staged kernels (64-ray blocks), beyond that the
+# direct ones; R = 4097 and 700 leave a partial last block.
+SHAPES = [(1, 1), (1000, 16), (4097, 7), (300, 64), (700, 51), (700, 52), (128, 3)]
+
+
+@pytest.mark.parametrize("R,S", SHAPES)
 @pytest.mark.parametrize("white", [False, True])
 def test_render_forward(dev, R, S, white):
     t, sigma, rgb = random_rays(R, S, 10 + S)
@@ -55,6 +60,29 @@ def test_render_backward(dev, white):
     torch_reference(torch.from_numpy(t).double(), st, ct, white).backward(torch.from_numpy(g).double())
     np.testing.assert_allclose(cd.grad.cpu().numpy(), ct.grad.numpy(), rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(sd.grad.cpu().numpy(), st.grad.numpy(), rtol=2e-4, atol=2e-6)
+
+
+@pytest.mark.parametrize("R,S", SHAPES)
+@pytest.mark.parametrize("fused", [False, True])
+def test_render_backward_shapes(dev, R, S, fused):
+    """render's backward with t (clamp_t fused) or with the intervals, on both
+    kernel forms, bit for bit against the oracle."""
+    if S == 1 and not fused:
+        pytest.skip("clamp_t of one sample gives no interval column")
+    t, sigma, rgb = random_rays(R, S, 30 + S)
+    g = np.random.default_rng(31 + S).normal(size=(R, 3)).astype(np.float32)
+    td, sd, cd, gd = (torch.from_numpy(a).to(dev) for a in (t, sigma, rgb, g))
+    print(f"render backward R={R} S={S} fused={fused}", flush=True)
+    if fused:
+        gs, gc = C.render_rays_backward(sd, cd, gd, white_bg=True, t_val=td)
+        ws, wc = oracle.render_backward(sigma, rgb, g, True, t=t)
+    else:
+        delta = oracle.clamp_t(t)
+        gs, gc = C.render_rays_backward(sd, cd, gd, white_bg=True, deltas=torch.from_numpy(delta).to(dev))
+        ws, wc = oracle.render_backward(sigma, rgb, g, True, delta=delta)
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(gs.cpu().numpy()), bits(ws))
+    assert np.array_equal(bits(gc.cpu().numpy()), bits(wc))
 
 
 def test_render_argument_errors(dev):
