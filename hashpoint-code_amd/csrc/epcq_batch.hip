@@ -453,7 +453,7 @@ extern "C" int epcq_batch_create(const epcq_options* opt, int32_t mode, int32_t 
     b->mode = mode;
     b->s_max = s_max;
     b->bound = bound;
-    epcq_tuning tu{32, 8, 1024, 0, kHeavyCandidates};
+    epcq_tuning tu{32, 8, 1024, 0, kHeavyCandidates, kListWaveRays};
     if (tuning) tu = *tuning;
     else epcq_get_tuning(&tu);
     b->tp = plan_tiles(opt->kernel_size, tu.tile_w, tu.tile_h, tu.tile_cap, tu.flags, radius != nullptr);

@@ -74,6 +74,7 @@ int check_tuning(const epcq_tuning& t, const char* who) {
         return fail(EPCQ_ERR_INVALID, w + ": tile_w must be a power of two <= 64 and tile_w * tile_h 64, 128 or 256");
     if (t.tile_cap < 0) return fail(EPCQ_ERR_INVALID, w + ": negative tile_cap");
     if (t.heavy < 0) return fail(EPCQ_ERR_INVALID, w + ": negative heavy");
+    if (t.list_wave < 0) return fail(EPCQ_ERR_INVALID, w + ": negative list_wave");
     if (t.flags & ~EPCQ_TUNE_XCD_ORDER) return fail(EPCQ_ERR_INVALID, w + ": unknown flags");
     return EPCQ_OK;
 }
@@ -83,7 +84,7 @@ namespace {
 constexpr int kBlock = 256;
 
 std::mutex g_tuning_mu;
-epcq_tuning g_tuning{32, 8, 1024, 0, kHeavyCandidates};
+epcq_tuning g_tuning{32, 8, 1024, 0, kHeavyCandidates, kListWaveRays};
 
 epcq_tuning current_tuning() {
     std::lock_guard<std::mutex> lk(g_tuning_mu);
@@ -234,7 +235,9 @@ SampleOpts make_opts(const float* xyz, const epcq_options* opt, const float* rad
     o.S = opt->num_sample_points_per_ray;
     o.Kc = opt->num_point_cloud_per_sp;
     o.xyz = xyz;
-    o.heavy = current_tuning().heavy;
+    const epcq_tuning tu = current_tuning();
+    o.heavy = tu.heavy;
+    o.list_wave = tu.list_wave;
     return o;
 }
 

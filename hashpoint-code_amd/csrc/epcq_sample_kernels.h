@@ -64,12 +64,37 @@ __global__ __launch_bounds__(kBlock) void sample_list_kernel(SampleOpts o, Sampl
     }
 }
 
+// One wave per listed ray (run_ray_wave, the same outputs as run_ray): for
+// short lists, such as a training batch, whose one-lane-per-ray form would
+// keep a few CUs busy while the rest of the GPU idles.
+template <int MODE, int KMAX, int SMAX, int LAY, bool PR, int NW>
+__global__ __launch_bounds__(kBlock) void sample_list_wave_kernel(SampleOpts o, SampleOut out, ViewArgs view,
+                                                                  const int* __restrict__ rays, int64_t R) {
+    const int64_t ri = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (ri >= R) return;  // the whole wave: ri is uniform
+    const SrcGlobal<PR> src{view.hash, view.sorted_idx, o.xyz, view.pts, view.prad, o.radius, view.cone_a,
+                            view.cone_b, view.W};
+    const int64_t q = rays[ri];
+    if (!(q >= 0 && q < (int64_t)view.W * view.H)) {
+        // a ray off the image: the empty outputs run_ray writes for it
+        if (__lane_id() == 0) write_tail<MODE, LAY>(o, out, ri, 0);
+        return;
+    }
+    const int u = (int)(q % view.W), v = (int)(q / view.W);
+    Ray ray;
+    view_ray(view, u, v, ray);
+    run_ray_wave<MODE, KMAX, SMAX, LAY, NW>(o, out, src, ray, u, v, view.W, view.H, ri);
+}
+
 }  // namespace
 
 template <int MODE, int KMAX, int SMAX, int LAY, bool PR, int NW>
 void launch_kernels(const SampleOpts& o, const SampleOut& out, const ViewArgs& view, const int* rays, int64_t R,
                 const TilePlan& tp, hipStream_t st) {
-    if (rays) {
+    if (rays && R <= o.list_wave) {
+        sample_list_wave_kernel<MODE, KMAX, SMAX, LAY, PR, NW>
+            <<<dim3((unsigned)ceil_div(R, kBlock / kWave)), dim3(kBlock), 0, st>>>(o, out, view, rays, R);
+    } else if (rays) {
         sample_list_kernel<MODE, KMAX, SMAX, LAY, PR, NW><<<dim3((unsigned)ceil_div(R, kBlock)), dim3(kBlock), 0, st>>>(
             o, out, view, rays, R);
     } else {

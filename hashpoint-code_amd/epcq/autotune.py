@@ -109,4 +109,4 @@ def tune(key, run, reps=3, stream=None, shapes=SHAPES, caps=CAPS, flags=FLAGS, h
 def apply(tuning):
     """Makes `tuning` the process-wide launch shape of the per-view samplers."""
     _C.set_tuning(tuning["tile_w"], tuning["tile_h"], tuning["tile_cap"], tuning.get("flags", 0),
-                  tuning.get("heavy", _C.HEAVY_DEFAULT))
+                  tuning.get("heavy", _C.HEAVY_DEFAULT), tuning.get("list_wave", _C.LIST_WAVE_DEFAULT))

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import csrc as _C
-from .csrc import HEAVY_DEFAULT, _BatchViewOut, _Tuning, _ViewDesc, _lib, _options_struct, _raise_on
+from .csrc import HEAVY_DEFAULT, LIST_WAVE_DEFAULT, _BatchViewOut, _Tuning, _ViewDesc, _lib, _options_struct, _raise_on
 
 
 def _host_floats(x, count):
@@ -78,7 +78,8 @@ class MultiViewSampler:
         tu = None
         if self.tuning is not None:
             tu = ctypes.byref(_Tuning(self.tuning["tile_w"], self.tuning["tile_h"], self.tuning["tile_cap"],
-                                      self.tuning.get("flags", 0), self.tuning.get("heavy", HEAVY_DEFAULT)))
+                                      self.tuning.get("flags", 0), self.tuning.get("heavy", HEAVY_DEFAULT),
+                                      self.tuning.get("list_wave", LIST_WAVE_DEFAULT)))
         h = ctypes.c_void_p(0)
         rad = ctypes.c_void_p(self.radius.data_ptr()) if self.radius is not None else ctypes.c_void_p(0)
         _raise_on(_lib.epcq_batch_create(ctypes.byref(self.opts), _C._MODES[self.mode], self.s_max, self.max_views,

@@ -58,7 +58,7 @@ class _Options(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32), ("tile_cap", ctypes.c_int32),
-                ("flags", ctypes.c_int32), ("heavy", ctypes.c_int32)]
+                ("flags", ctypes.c_int32), ("heavy", ctypes.c_int32), ("list_wave", ctypes.c_int32)]
 
 
 TUNE_XCD_ORDER = 1
@@ -487,21 +487,24 @@ def quick_query_for_nearby_point(neural_points, cam, options, sorted_point_idx_l
 
 
 HEAVY_DEFAULT = 128
+LIST_WAVE_DEFAULT = 65536
 
 
-def set_tuning(tile_w=32, tile_h=8, tile_cap=1024, flags=0, heavy=HEAVY_DEFAULT):
+def set_tuning(tile_w=32, tile_h=8, tile_cap=1024, flags=0, heavy=HEAVY_DEFAULT, list_wave=LIST_WAVE_DEFAULT):
     """Launch shape of the full-image samplers: tile_w x tile_h rays per
     block, tile_cap points staged in LDS per tile, TUNE_* flags; rays whose
     unculled window buckets hold more than `heavy` points are sampled by a
-    whole wave (0: never)."""
+    whole wave (0: never); ray lists of at most `list_wave` rays are sampled
+    one wave per ray (0: never)."""
     _raise_on(_lib.epcq_set_tuning(ctypes.byref(_Tuning(int(tile_w), int(tile_h), int(tile_cap), int(flags),
-                                                        int(heavy)))))
+                                                        int(heavy), int(list_wave)))))
 
 
 def get_tuning():
     t = _Tuning()
     _raise_on(_lib.epcq_get_tuning(ctypes.byref(t)))
-    return {"tile_w": t.tile_w, "tile_h": t.tile_h, "tile_cap": t.tile_cap, "flags": t.flags, "heavy": t.heavy}
+    return {"tile_w": t.tile_w, "tile_h": t.tile_h, "tile_cap": t.tile_cap, "flags": t.flags, "heavy": t.heavy,
+            "list_wave": t.list_wave}
 
 
 def squared_threshold(r):

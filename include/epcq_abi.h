@@ -77,9 +77,11 @@ const char* epcq_last_error(void);
  * unculled window buckets hold more than this many points (more than 4x this
  * many when over 16 rays of its wave of 64 exceed it) is sampled by a whole
  * wave (64 lanes share its candidate walks) after the tile's other rays,
- * instead of by one lane; 0 never does.  Results are identical either way.
- * Process-wide, guarded by a mutex; the autotuner sets it.  Defaults:
- * 32, 8, 1024, 0, 128.
+ * instead of by one lane; 0 never does.  list_wave: a ray list (the rays
+ * argument of the samplers) of at most this many rays is sampled one wave per
+ * ray, longer ones one lane per ray; 0 never does.  Results are identical
+ * either way.  Process-wide, guarded by a mutex; the autotuner sets it.
+ * Defaults: 32, 8, 1024, 0, 128, 65536.
  */
 typedef struct epcq_tuning {
     int32_t tile_w;
@@ -87,6 +89,7 @@ typedef struct epcq_tuning {
     int32_t tile_cap;
     int32_t flags;
     int32_t heavy;
+    int32_t list_wave;
 } epcq_tuning;
 #define EPCQ_TUNE_XCD_ORDER 1
 int epcq_set_tuning(const epcq_tuning* t);

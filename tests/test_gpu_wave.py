@@ -13,11 +13,25 @@ from epcq import synth
 from epcq.batch import MultiViewSampler
 from epcq.epcq import RenderOptions
 from oracle import oracle
-from scenes import to_dev
+from scenes import bits, to_dev
 from test_gpu_batch import KEYS, compare, per_view
 from test_gpu_sampling import View, check_dense, compact_vs_dense
 
 pytestmark = pytest.mark.gpu
+
+
+class tuning:
+    """Process-wide tuning fields for the duration of a with-block."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+
+    def __enter__(self):
+        self.old = C.get_tuning()
+        C.set_tuning(**dict(self.old, **self.kw))
+
+    def __exit__(self, *exc):
+        C.set_tuning(**self.old)
 
 
 class heavy_threshold:
@@ -121,3 +135,50 @@ def test_wave_form_batch(dev, mode):
     assert np.array_equal(o["n_samples"].cpu().numpy(), res[3].sum(1))
     assert np.array_equal(o["nb_idx"].cpu().numpy(), res[1][:, :s])
     assert np.array_equal(o["sample_t"].cpu().numpy().view(np.int32), res[2][:, :s].view(np.int32))
+
+
+def dense_rays(v, mode, uv):
+    """The dense drop-in layout for a training ray subset (pixel coordinates
+    uv [R, 2]) against the oracle."""
+    sp_xyz, pc_idx, sp_t, mask = v.npts.render_image_hashtable_topk(
+        v.cs, v.opt, v.idx_d, None, v.hash_d, mode=mode, training_ray_index=to_dev(uv, v.dev))
+    torch.cuda.synchronize()
+    rays = (uv[:, 1] * v.cam.width + uv[:, 0]).astype(np.int32)
+    o_xyz, o_idx, o_t, o_mask = oracle.quick_sampling(v.pts, v.ocam, v.oopt, v.oi, v.oh, rays=rays,
+                                                      mode=oracle.MODE_KNN if mode == "knn" else oracle.MODE_COMPAT)
+    np.testing.assert_array_equal(mask.cpu().numpy().reshape(o_mask.shape), o_mask)
+    np.testing.assert_array_equal(pc_idx.cpu().numpy().reshape(o_idx.shape), o_idx)
+    assert np.array_equal(bits(sp_t.cpu().numpy().reshape(o_t.shape)), bits(o_t))
+    assert np.array_equal(bits(sp_xyz.cpu().numpy().reshape(o_xyz.shape)), bits(o_xyz))
+    assert o_mask.sum() > 1000
+
+
+@pytest.mark.parametrize("form", ["lane", "wave"])
+def test_ray_list_forms(dev, form):
+    """Ray lists sampled one lane per ray (list_wave = 0, no deferral) and one
+    wave per ray (list_wave above the list's length), each bit for bit
+    against the oracle: the dense drop-in layout of a training batch, the
+    compact layout with fp32 and bf16 features and rays off the image, and
+    per-point radii with a 121-lane window (4-word lane sets)."""
+    lanes = form == "lane"
+    knobs = dict(list_wave=0, heavy=0) if lanes else dict(list_wave=1 << 30)
+    pts, cam = synth.box_scene(400_000, seed=7), synth.look_at_camera([-2.0, -2.5, 2.0], 400, 300, 555.5)
+    v = View(pts, cam, dev, radius_threshold=0.008, num_point_cloud_per_sp=8)
+    rng = np.random.default_rng(11)
+    uv = np.stack([rng.integers(0, cam.width, 4096), rng.integers(0, cam.height, 4096)], axis=1).astype(np.int64)
+    rays = rng.integers(-5, cam.width * cam.height + 5, size=5000).astype(np.int32)
+    feat = torch.randn(len(pts), 16, generator=torch.Generator().manual_seed(12)).to(dev)
+    print(f"ray list form={form}", flush=True)
+    with tuning(**knobs):
+        dense_rays(v, "compat", uv)
+        dense_rays(v, "knn", uv)
+        compact_vs_dense(v, "knn", rays=rays, feat=feat)
+        compact_vs_dense(v, "compat", rays=rays, feat=feat[:, :13].contiguous().to(torch.bfloat16))
+    pts = synth.box_scene(300_000, n_boxes=40, seed=8)
+    radius = synth.log_uniform_radius(len(pts), seed=9)
+    cam = synth.look_at_camera([2.0, -2.0, 3.0], 320, 180, 444.0)
+    v = View(pts, cam, dev, radius=radius, kernel_size=11, radius_threshold=0.01, num_point_cloud_per_sp=16)
+    rays = rng.integers(0, cam.width * cam.height, size=3000).astype(np.int32)
+    print(f"ray list form={form}: per-point radius, k = 11, K = 16", flush=True)
+    with tuning(**knobs):
+        compact_vs_dense(v, "knn", rays=rays)

@@ -88,11 +88,15 @@ def test_tuning_roundtrip_and_validation():
     try:
         C.set_tuning(32, 4, 1024, C.TUNE_XCD_ORDER, heavy=7)
         assert C.get_tuning() == {"tile_w": 32, "tile_h": 4, "tile_cap": 1024, "flags": C.TUNE_XCD_ORDER,
-                                  "heavy": 7}
+                                  "heavy": 7, "list_wave": C.LIST_WAVE_DEFAULT}
+        C.set_tuning(32, 4, 1024, list_wave=0)
+        assert C.get_tuning()["list_wave"] == 0
         with pytest.raises(RuntimeError, match="unknown flags"):
             C.set_tuning(16, 16, 1024, flags=2)
         with pytest.raises(RuntimeError, match="heavy"):
             C.set_tuning(16, 16, 1024, heavy=-1)
+        with pytest.raises(RuntimeError, match="list_wave"):
+            C.set_tuning(16, 16, 1024, list_wave=-1)
         with pytest.raises(RuntimeError, match="tile_w"):
             C.set_tuning(12, 8, 1024)
         with pytest.raises(RuntimeError, match="tile_w"):
