@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define EPCQ_ABI_VERSION 2
+#define EPCQ_ABI_VERSION 3
 
 enum epcq_status {
     EPCQ_OK = 0,

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     assert sorted(C.ABI_SYMBOLS) == names
     for n in names:
         assert getattr(C._lib, n) is not None
-    assert C._lib.epcq_abi_version() == 2
+    assert C._lib.epcq_abi_version() == 3
 
 
 def test_squared_threshold_is_exact():
