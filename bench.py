@@ -753,7 +753,7 @@ def bench_render(args):
         "config": {"workload": "render-800x800-S16", "rays": R, "samples_per_ray": S, "white_bg": True},
         "backward": {"ms": bwd, "rays_per_s": R / (bwd * 1e-3), "achieved_GBs": bb / (bwd * 1e-3) / 1e9,
                      "bytes_per_launch": bb},
-        "roofline": {"bound": "hbm", "kernel": "render_staged_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": "render_vec_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": fb,
                      "bytes_model": "per ray 4S (t) + 4S (sigma) + 12S (rgb) + 12 (colour)"},
         "torch_fp32_gpu": {"ms_per_step": ref, "speedup": ref / fwd},

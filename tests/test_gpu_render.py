@@ -91,3 +91,39 @@ def test_render_argument_errors(dev):
         C.render_rays(z, torch.zeros((4, 3, 3), device=dev))
     with pytest.raises(RuntimeError, match="rgbs must have shape"):
         C.render_rays(z, torch.zeros((4, 3), device=dev), deltas=z)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_render_unaligned_rows(dev, fused):
+    """Rows whose arrays start 4 bytes past a 16-byte boundary (views into a
+    larger buffer) take the kernels that load one float at a time; the
+    results are the same bit for bit."""
+    R, S = 3000, 16
+    t, sigma, rgb = random_rays(R, S, 41)
+    g = np.random.default_rng(42).normal(size=(R, 3)).astype(np.float32)
+
+    def shifted(a):
+        buf = torch.empty(a.size + 1, device=dev)
+        buf[1:] = torch.from_numpy(a.reshape(-1)).to(dev)
+        return buf[1:].view(a.shape)
+
+    td, sd, cd = shifted(t), shifted(sigma), shifted(rgb)
+    assert sd.data_ptr() % 16 == 4
+    gd = torch.from_numpy(g).to(dev)
+    print(f"render unaligned fused={fused}", flush=True)
+    if fused:
+        out = C.render_rays(sd, cd, white_bg=True, t_val=td)
+        gs, gc = C.render_rays_backward(sd, cd, gd, white_bg=True, t_val=td)
+        ref = oracle.render(sigma, rgb, True, t=t)
+        ws, wc = oracle.render_backward(sigma, rgb, g, True, t=t)
+    else:
+        delta = oracle.clamp_t(t)
+        dd = shifted(delta)
+        out = C.render_rays(sd, cd, white_bg=True, deltas=dd)
+        gs, gc = C.render_rays_backward(sd, cd, gd, white_bg=True, deltas=dd)
+        ref = oracle.render(sigma, rgb, True, delta=delta)
+        ws, wc = oracle.render_backward(sigma, rgb, g, True, delta=delta)
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(out.cpu().numpy()), bits(ref))
+    assert np.array_equal(bits(gs.cpu().numpy()), bits(ws))
+    assert np.array_equal(bits(gc.cpu().numpy()), bits(wc))
