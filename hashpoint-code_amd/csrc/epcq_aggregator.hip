@@ -74,6 +74,15 @@ __device__ __forceinline__ void pe_sincos(float x, float& s, float& c) {
     }
 }
 
+// sin and cos of 2x from those of x (double-angle formulas).  Each step at
+// most quadruples the absolute error of its inputs and adds a rounding; the
+// encodings take at most two steps from a pe_sincos value, which keeps them
+// within 16 times its error, far below the bf16 step they are rounded to.
+__device__ __forceinline__ void pe_double(float s, float c, float& s2, float& c2) {
+    s2 = 2.0f * s * c;
+    c2 = c * c - s * s;
+}
+
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ float lrelu(float v) { return v > 0.0f ? v : 0.01f * v; }
@@ -265,8 +274,9 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
     for (int i = 0; i < 16; ++i) {
         // PE(f, 3) of feature d = half * 16 + i: 6 values at 32 + 6 d
         float sn[3], cs[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) pe_sincos(fs[i] * (float)(1 << j), sn[j], cs[j]);
+        pe_sincos(fs[i], sn[0], cs[0]);
+        pe_double(sn[0], cs[0], sn[1], cs[1]);
+        pe_double(sn[1], cs[1], sn[2], cs[2]);
         uint32_t* o = reinterpret_cast<uint32_t*>(xr + kFeat + 6 * (half * 16 + i));
         o[0] = pack_bf16(sn[0], cs[0]);
         o[1] = pack_bf16(sn[1], cs[1]);
@@ -279,12 +289,15 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
         uint32_t* o = reinterpret_cast<uint32_t*>(xr + 224);
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
+            // frequencies 1 and 4 directly; 2, 8 and 16 by doubling
+            float sn[5], cs[5];
+            pe_sincos(df[i], sn[0], cs[0]);
+            pe_sincos(df[i] * 4.0f, sn[2], cs[2]);
+            pe_double(sn[0], cs[0], sn[1], cs[1]);
+            pe_double(sn[2], cs[2], sn[3], cs[3]);
+            pe_double(sn[3], cs[3], sn[4], cs[4]);
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                float sn, cs;
-                pe_sincos(df[i] * (float)(1 << j), sn, cs);
-                o[i * 5 + j] = pack_bf16(sn, cs);
-            }
+            for (int j = 0; j < 5; ++j) o[i * 5 + j] = pack_bf16(sn[j], cs[j]);
         }
         o[30] = 0;  // columns 284..287
         o[31] = 0;
