@@ -2,7 +2,7 @@
 // render (epcq/epcq.py:508-530, also utils/utils.py:422-467), as used after
 // the Aggregator (opt/render_image.py:101-109), and render's backward.
 //
-// One lane per ray walks its S samples front to back in registers:
+// One lane per ray walks its S samples front to back:
 //   delta_i = t_{i+1} - t_i (vsize for the last), replaced by vsize where
 //             below 1e-8 or above 2 vsize                   (clamp_t)
 //   op_i    = 1 - exp(-sigma_i delta_i)
@@ -13,8 +13,9 @@
 // one-sample shift.  With S = 1 the reference's clamp_t returns no interval
 // at all (its appended vsize column is cut from an empty slice), so no
 // sample contributes and the colour is the background: the fused form (t
-// given) does the same, and clamp_t writes nothing.  Every operation is the reference's, one rounding each,
-// in this order; the exponential is the library's own (expf_cw).  The
+// given) does the same, and clamp_t writes nothing.  Every operation is the
+// reference's, one rounding each, in this order; the exponential is the
+// library's own (expf_cw).  The
 // backward pass recomputes the walk and forms, per sample k (g = dL/drgb):
 //   dL/dc_k     = op_k T_k g
 //   dL/dsigma_k = delta_k e_k (T_k (g . c_k) - (R_k + [k < S-1] T_{S-1} (g . bg)) / a_k)
@@ -80,7 +81,8 @@ __device__ __forceinline__ void load_chunk(const float* tr, const float* dr, con
         const float tnext = k0 + n < S ? tr[k0 + n] : 0.0f;  // the depth after the chunk
 #pragma unroll
         for (int j = 0; j < CH; ++j)
-            if (j < n) ch.d[j] = clamp_delta_v(tv[j], j + 1 < n ? tv[min(j + 1, CH - 1)] : tnext, k0 + j + 1 >= S, vsize);
+            if (j < n)
+                ch.d[j] = clamp_delta_v(tv[j], j + 1 < n ? tv[min(j + 1, CH - 1)] : tnext, k0 + j + 1 >= S, vsize);
     }
 }
 
@@ -191,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(const float* __restrict_
     const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (r >= R) return;
     float o0, o1, o2;
-    render_row<kChunk>(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S, rgb + r * S * 3, S, vsize, bg,
-               o0, o1, o2);
+    render_row<kChunk>(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S, rgb + r * S * 3, S,
+                       vsize, bg, o0, o1, o2);
     out[3 * r] = o0;
     out[3 * r + 1] = o1;
     out[3 * r + 2] = o2;
@@ -204,9 +206,9 @@ __global__ __launch_bounds__(kBlock) void render_backward_kernel(
     float* __restrict__ grad_sigma, float* __restrict__ grad_rgb) {
     const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (r >= R) return;
-    render_row_backward<kChunk>(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S, rgb + r * S * 3, S,
-                        vsize, bg, grad_out[3 * r], grad_out[3 * r + 1], grad_out[3 * r + 2], grad_sigma + r * S,
-                        grad_rgb + r * S * 3);
+    render_row_backward<kChunk>(t ? t + r * S : nullptr, delta ? delta + r * S : nullptr, sigma + r * S,
+                                rgb + r * S * 3, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
+                                grad_out[3 * r + 2], grad_sigma + r * S, grad_rgb + r * S * 3);
 }
 
 // Staged forms, for rows that fit: a one-wave block takes 64 consecutive rays,
@@ -310,8 +312,8 @@ __global__ __launch_bounds__(kStageRays) void render_staged_kernel(const float* 
     const int l = threadIdx.x;
     if (l >= nr) return;
     float o0, o1, o2;
-    render_row<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws, sc + l * wc, S, vsize, bg, o0, o1,
-               o2);
+    render_row<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws, sc + l * wc, S, vsize,
+                          bg, o0, o1, o2);
     const int64_t r = r0 + l;
     out[3 * r] = o0;
     out[3 * r + 1] = o1;
@@ -335,8 +337,9 @@ __global__ __launch_bounds__(kStageRays) void render_backward_staged_kernel(
     if (l < nr) {
         const int64_t r = r0 + l;
         // the gradients replace the row's sigma and rgb in LDS
-        render_row_backward<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws, sc + l * wc, S, vsize,
-                            bg, grad_out[3 * r], grad_out[3 * r + 1], grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
+        render_row_backward<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
+                                       sc + l * wc, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
+                                       grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
     }
     __syncthreads();
     unstage_rows(grad_sigma + r0 * S, ss, nr, S, ws);
