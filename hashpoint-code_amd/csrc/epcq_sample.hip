@@ -251,12 +251,21 @@ ViewArgs make_view(const epcq_camera* cam, const int* sorted_idx, const int* has
 size_t cone_bytes(int64_t P) { return align_up((size_t)P * 16, 256) + align_up((size_t)P * 8, 256); }
 
 // Bucket cones and the pixel-ordered point copy in the workspace (when it is
-// large enough: then o.cull is set and view reads them).
+// large enough: then o.cull is set and view reads them).  They cost a pass
+// over every pixel and point of the view, so a ray list (rays >= 0 rays)
+// whose windows visit fewer than P / kConeVisitRatio buckets in all skips
+// them.  Measured: a 4096-ray batch of a 1M-point 800x800 view with k = 5
+// (visits 0.16 P) samples in 0.069 ms without them against 0.080 ms with,
+// and both ways meet near 0.24 P; on a 20M-point 1080p view with k = 15 they
+// meet near 0.2 P.  The outputs are the same either way.
+constexpr int64_t kConeVisitRatio = 4;
+
 void prepare_bounds(SampleOpts& o, ViewArgs& view, int64_t n, const epcq_camera* cam, const int* sorted_idx,
-                    const int* hash, void* workspace, size_t workspace_bytes, hipStream_t st) {
+                    const int* hash, void* workspace, size_t workspace_bytes, int64_t rays, hipStream_t st) {
     const int64_t P = (int64_t)cam->width * cam->height;
     o.cull = 0;
     if (!workspace || workspace_bytes < cone_bytes(P)) return;
+    if (rays >= 0 && rays * o.L * kConeVisitRatio < P) return;
     char* w = static_cast<char*>(workspace);
     float4* ca = reinterpret_cast<float4*>(w);
     float2* cb = reinterpret_cast<float2*>(w + align_up((size_t)P * 16, 256));
@@ -387,7 +396,7 @@ extern "C" int epcq_quick_sampling(const float* xyz, int64_t n, const epcq_camer
     out.mask = mask;
     hipStream_t st = as_stream(stream);
     ViewArgs view = make_view(cam, sorted_idx, hash);
-    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, -1, st);
     return launch_sample(o, out, view, nullptr, (int64_t)cam->width * cam->height, mode, bound, st,
                          "quick_sampling");
 }
@@ -415,7 +424,7 @@ extern "C" int epcq_quick_sampling_rays(const float* xyz, int64_t n, const epcq_
     out.mask = mask;
     hipStream_t st = as_stream(stream);
     ViewArgs view = make_view(cam, sorted_idx, hash);
-    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, R, st);
     return launch_sample(o, out, view, rays, R, mode, bound, st, "quick_sampling");
 }
 
@@ -453,7 +462,7 @@ extern "C" int epcq_knn_sampling(const float* xyz, int64_t n, const epcq_camera*
     const int64_t nr = rays ? R : (int64_t)cam->width * cam->height;
     hipStream_t st = as_stream(stream);
     ViewArgs view = make_view(cam, sorted_idx, hash);
-    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, st);
+    prepare_bounds(o, view, n, cam, sorted_idx, hash, workspace, workspace_bytes, rays ? R : -1, st);
     return launch_sample(o, out, view, rays, nr, mode, bound, st, "knn_sampling");
 }
 

@@ -137,7 +137,7 @@ def test_wave_form_batch(dev, mode):
     assert np.array_equal(o["sample_t"].cpu().numpy().view(np.int32), res[2][:, :s].view(np.int32))
 
 
-def dense_rays(v, mode, uv):
+def dense_rays(v, mode, uv, min_live=1000):
     """The dense drop-in layout for a training ray subset (pixel coordinates
     uv [R, 2]) against the oracle."""
     sp_xyz, pc_idx, sp_t, mask = v.npts.render_image_hashtable_topk(
@@ -150,7 +150,7 @@ def dense_rays(v, mode, uv):
     np.testing.assert_array_equal(pc_idx.cpu().numpy().reshape(o_idx.shape), o_idx)
     assert np.array_equal(bits(sp_t.cpu().numpy().reshape(o_t.shape)), bits(o_t))
     assert np.array_equal(bits(sp_xyz.cpu().numpy().reshape(o_xyz.shape)), bits(o_xyz))
-    assert o_mask.sum() > 1000
+    assert o_mask.sum() > min_live
 
 
 @pytest.mark.parametrize("form", ["lane", "wave"])
@@ -158,8 +158,9 @@ def test_ray_list_forms(dev, form):
     """Ray lists sampled one lane per ray (list_wave = 0, no deferral) and one
     wave per ray (list_wave above the list's length), each bit for bit
     against the oracle: the dense drop-in layout of a training batch, the
-    compact layout with fp32 and bf16 features and rays off the image, and
-    per-point radii with a 121-lane window (4-word lane sets)."""
+    compact layout with fp32 and bf16 features and rays off the image, short
+    lists that sample without bucket cones, and per-point radii with a
+    121-lane window (4-word lane sets)."""
     lanes = form == "lane"
     knobs = dict(list_wave=0, heavy=0) if lanes else dict(list_wave=1 << 30)
     pts, cam = synth.box_scene(400_000, seed=7), synth.look_at_camera([-2.0, -2.5, 2.0], 400, 300, 555.5)
@@ -174,6 +175,10 @@ def test_ray_list_forms(dev, form):
         dense_rays(v, "knn", uv)
         compact_vs_dense(v, "knn", rays=rays, feat=feat)
         compact_vs_dense(v, "compat", rays=rays, feat=feat[:, :13].contiguous().to(torch.bfloat16))
+        # lists this short (64 rays x 25 window pixels, under a quarter of the
+        # view's pixels) skip the bucket cones and the pixel-ordered copy
+        dense_rays(v, "compat", uv[:64], min_live=5)
+        compact_vs_dense(v, "knn", rays=rays[:64], feat=feat)
     pts = synth.box_scene(300_000, n_boxes=40, seed=8)
     radius = synth.log_uniform_radius(len(pts), seed=9)
     cam = synth.look_at_camera([2.0, -2.0, 3.0], 320, 180, 444.0)
