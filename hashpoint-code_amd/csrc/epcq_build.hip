@@ -11,10 +11,10 @@
 //   1. count    each 256-thread block projects a chunk of points and builds an
 //               LDS histogram over coarse bins (runs of 2^ts consecutive
 //               row-major pixels);
-//   2. scan     per-bin column scan of the block histograms (four small
-//               kernels: segment sums, bin totals, bin starts, cursors)
-//               gives every (block, bin) a write cursor and every bin its
-//               start in the output;
+//   2. scan     one block per bin scans its column of the block histograms
+//               (each (block, bin) gets its cursor within the bin, each bin
+//               its total), then one block scans the totals into the bins'
+//               starts in the output, which the bin pass adds;
 //   3. bin      blocks re-project their chunk and append each point to its
 //               bin; lanes with equal bins are matched with ballots and waves
 //               take their turn in order, so each bin receives its points in
@@ -40,22 +40,15 @@ __global__ __launch_bounds__(kBlock) void build_count_kernel(BuildView bv) {
     count_body(bv, blockIdx.x, s_hist);
 }
 
-__global__ __launch_bounds__(kBlock) void col_partial_kernel(BuildView bv) {
-    col_partial_body(bv, blockIdx.x * kBlock + threadIdx.x, blockIdx.y);
-}
-
-__global__ __launch_bounds__(kBlock) void bin_total_kernel(BuildView bv) {
-    bin_total_body(bv, blockIdx.x * kBlock + threadIdx.x);
+__global__ __launch_bounds__(kBlock) void col_scan_kernel(BuildView bv) {
+    __shared__ int s_wave[kBuildWaves];
+    col_scan_body(bv, blockIdx.x, s_wave);
 }
 
 __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BuildView bv) {
     __shared__ int s_wave[kBuildWaves];
     __shared__ int s_carry;
     bin_scan_body(bv, s_wave, &s_carry);
-}
-
-__global__ __launch_bounds__(kBlock) void col_apply_kernel(BuildView bv) {
-    col_apply_body(bv, blockIdx.x * kBlock + threadIdx.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(kBlock) void build_bin_kernel(BuildView bv) {
@@ -226,19 +219,16 @@ int build_hashtable_impl(const float* xyz, int64_t n, const float* w2c, const fl
         return fail(EPCQ_ERR_WORKSPACE, w + ": workspace too small");
     hipStream_t st = as_stream(stream);
     if (n == 0) return fill_launch(out_hash, 2 * bp.P, -1, st);
-    const BuildView bv = make_build_view(bp, xyz, n, w2c, K, height, width, sort, workspace, out_idx, out_depth,
-                                         out_hash);
+    BuildView bv = make_build_view(bp, xyz, n, w2c, K, height, width, sort, workspace, out_idx, out_depth,
+                                   out_hash);
+    bv.rel_cursors = 1;  // one block per bin scans its column; binning adds the bin's start
     const size_t lds_bins = (size_t)bp.nbins * 4;
     build_count_kernel<<<dim3(bp.nblk), dim3(kBlock), lds_bins, st>>>(bv);
     if (int e = check_launch("build_hashtable/count")) return e;
-    const dim3 cgrid((unsigned)ceil_div(bp.nbins, kBlock), (unsigned)bp.nseg);
-    col_partial_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);
-    if (int e = check_launch("build_hashtable/col_partial")) return e;
-    bin_total_kernel<<<dim3(cgrid.x), dim3(kBlock), 0, st>>>(bv);
+    col_scan_kernel<<<dim3(bp.nbins), dim3(kBlock), 0, st>>>(bv);
+    if (int e = check_launch("build_hashtable/col_scan")) return e;
     bin_scan_kernel<<<dim3(1), dim3(kBlock), 0, st>>>(bv);
     if (int e = check_launch("build_hashtable/bin_scan")) return e;
-    col_apply_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);
-    if (int e = check_launch("build_hashtable/col_apply")) return e;
     build_bin_kernel<<<dim3(bp.nblk), dim3(kBlock), lds_bins, st>>>(bv);
     if (int e = check_launch("build_hashtable/bin")) return e;
     const size_t lds_fine = (size_t)2 * (size_t(1) << bp.ts) * 4;

@@ -262,6 +262,7 @@ struct BuildView {
     int64_t chunk;
     int nblk, nseg;
     int sort;
+    int rel_cursors;  // hist holds cursors relative to the bin's start (bstart is added when binning)
     int* hist;    // [nblk][nbins] counts, then write cursors
     int* part;    // [nseg][nbins]
     int* bstart;  // [nbins + 1]
@@ -428,6 +429,25 @@ __device__ __forceinline__ void bin_scan_body(const BuildView& bv, int* s_wave, 
     __syncthreads();
 }
 
+// One block per bin b: hist[blk][b] <- the exclusive prefix of bin b's
+// column over the blocks (the cursor relative to the bin's start), btotal[b]
+// <- the column's total.  Thread t scans a run of ceil(nblk / 256) blocks.
+__device__ __forceinline__ void col_scan_body(const BuildView& bv, int b, int* s_wave) {
+    const int per = (bv.nblk + kBuildBlock - 1) / kBuildBlock;
+    const int j0 = (int)threadIdx.x * per, j1 = min(j0 + per, bv.nblk);
+    int sum = 0;
+    for (int j = j0; j < j1; ++j) sum += bv.hist[(int64_t)j * bv.nbins + b];
+    int total;
+    int run = block_exclusive_scan(sum, s_wave, total);
+    for (int j = j0; j < j1; ++j) {
+        const int64_t at = (int64_t)j * bv.nbins + b;
+        const int v = bv.hist[at];
+        bv.hist[at] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) bv.btotal[b] = total;
+}
+
 // hist[blk][b] <- absolute write cursor of block blk in bin b.
 __device__ __forceinline__ void col_apply_body(const BuildView& bv, int b, int seg) {
     if (b >= bv.nbins) return;
@@ -446,7 +466,8 @@ __device__ __forceinline__ void bin_body(const BuildView& bv, int blk, int* s_cu
     ProjParams pp;
     load_proj(bv.w2c, bv.K, bv.H, bv.W, pp);
     const int* row = bv.hist + (int64_t)blk * bv.nbins;
-    for (int b = threadIdx.x; b < bv.nbins; b += kBuildBlock) s_cur[b] = row[b];
+    for (int b = threadIdx.x; b < bv.nbins; b += kBuildBlock)
+        s_cur[b] = row[b] + (bv.rel_cursors ? bv.bstart[b] : 0);
     __syncthreads();
     const int64_t base = (int64_t)blk * bv.chunk;
     const int64_t end = min(base + bv.chunk, bv.n);
