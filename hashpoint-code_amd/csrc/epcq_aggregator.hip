@@ -85,7 +85,9 @@ __device__ __forceinline__ void pe_double(float s, float c, float& s2, float& c2
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float lrelu(float v) { return v > 0.0f ? v : 0.01f * v; }
+// v for v > 0, else 0.01 v (both branches agree on +-0, infinities and NaN):
+// one multiply and one max instead of a compare and a select as well.
+__device__ __forceinline__ float lrelu(float v) { return fmaxf(v, 0.01f * v); }
 
 // torch.nn.Softplus(beta=1, threshold=20)
 __device__ __forceinline__ float softplus(float v) { return v > 20.0f ? v : log1pf(expf(v)); }
