@@ -54,8 +54,11 @@ constexpr int kX1 = 288, kH = 256, kX3 = 272, kXc = 288, kC = 128;
 // Round to nearest even (the hardware conversion, v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
+// Two floats rounded to bf16 by one packed conversion, lo in the low half.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
-    return (uint32_t)to_bf16(lo) | ((uint32_t)to_bf16(hi) << 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 
 // sin and cos of x for the positional encodings, whose results are rounded
@@ -146,11 +149,14 @@ __device__ __forceinline__ void strip_layer(uint16_t* io, const uint16_t* __rest
     for (int t = 0; t < NT; ++t) {
         const int c = t * 32 + row;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int r = 0; r < 16; r += 2) {
+            // rows rr and rr + 1 of column c: one conversion for both
             const int rr = (r >> 2) * 8 + r0 + (r & 3);
-            float v = acc[t][r];
-            if (LEAKY) v = lrelu(v);
-            io[rr * kLdsStride + c] = to_bf16(v);
+            float v0 = acc[t][r], v1 = acc[t][r + 1];
+            if (LEAKY) v0 = lrelu(v0), v1 = lrelu(v1);
+            const uint32_t p = pack_bf16(v0, v1);
+            io[rr * kLdsStride + c] = (uint16_t)p;
+            io[(rr + 1) * kLdsStride + c] = (uint16_t)(p >> 16);
         }
     }
 }
@@ -202,9 +208,12 @@ __device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const u
     for (int t = 0; t < NT; ++t) {
         const int c = t * 32 + row;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int r = 0; r < 16; r += 2) {
+            // rows rr and rr + 1 of column c: one conversion for both
             const int rr = (r >> 2) * 8 + r0 + (r & 3);
-            io[rr * kLdsStride + c] = to_bf16(lrelu(acc[t][r]));
+            const uint32_t p = pack_bf16(lrelu(acc[t][r]), lrelu(acc[t][r + 1]));
+            io[rr * kLdsStride + c] = (uint16_t)p;
+            io[(rr + 1) * kLdsStride + c] = (uint16_t)(p >> 16);
         }
     }
 }
