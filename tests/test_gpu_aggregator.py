@@ -95,3 +95,20 @@ def test_repacks_after_a_parameter_update(dev):
         a1, _ = C.aggregator_forward(mod.packed_params(), *inp)
     live = inp[5].any(-1)
     assert (a1[live] > a0[live]).all()
+
+
+def test_encodings_of_large_features(dev):
+    """Features of magnitude 1024 to 4096 take the library's sincosf for the
+    base frequency (and double-angle steps from it); the raw-feature columns
+    of the first layer are zeroed so that the encodings carry the inputs."""
+    mod = module(dev, 11)
+    with torch.no_grad():
+        mod.block1[0].weight[:, :32] = 0.0
+    pc, rgb, d, feat, sp, valid, rays = random_inputs(500, 8, seed=12)
+    g = torch.Generator().manual_seed(13)
+    big = torch.rand(feat.shape, generator=g) * 3072 + 1024
+    big = big * torch.where(torch.rand(feat.shape, generator=g) < 0.5, -1.0, 1.0)
+    feat = torch.where(torch.rand((500, 1, 1), generator=g) < 0.5, big, feat).masked_fill(~valid[..., None], 0.0)
+    assert (feat.abs() >= 1024).any()
+    print("aggregator large features", flush=True)
+    check(mod, (pc, rgb, d, feat, sp, valid, rays), dev)
