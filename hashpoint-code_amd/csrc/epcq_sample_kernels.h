@@ -77,13 +77,13 @@ __global__ __launch_bounds__(kBlock) void sample_list_wave_kernel(SampleOpts o, 
     const int64_t q = rays[ri];
     if (!(q >= 0 && q < (int64_t)view.W * view.H)) {
         // a ray off the image: the empty outputs run_ray writes for it
-        if (__lane_id() == 0) write_tail<MODE, LAY>(o, out, ri, 0);
+        write_tail_wave<MODE, LAY>(o, out, ri, 0);
         return;
     }
     const int u = (int)(q % view.W), v = (int)(q / view.W);
     Ray ray;
     view_ray(view, u, v, ray);
-    run_ray_wave<MODE, KMAX, SMAX, LAY, NW>(o, out, src, ray, u, v, view.W, view.H, ri);
+    run_ray_wave<MODE, KMAX, SMAX, LAY, NW, true>(o, out, src, ray, u, v, view.W, view.H, ri);
 }
 
 }  // namespace

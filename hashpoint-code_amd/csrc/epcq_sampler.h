@@ -1138,12 +1138,21 @@ __device__ __forceinline__ void write_neighbours_wave(const SampleOpts& o, const
     }
 }
 
-// write_tail of the feature layout with the wave's lanes sharing the slots.
-template <int MODE>
+// write_tail with the wave's lanes sharing the slots: lane l writes elements
+// l, l + 64, ... of each output range the tail covers.
+template <int MODE, int LAY>
 __device__ __forceinline__ void write_tail_wave(const SampleOpts& o, const SampleOut& out, int64_t ri, int ns) {
-    constexpr int LAY = kLayFeat;
     const int lane = __lane_id();
-    {
+    if constexpr (LAY == kLayDense) {
+        const int64_t a0 = ri * o.S + ns;
+        const int n = o.S - ns;  // empty slots
+        for (int i = lane; i < 3 * n; i += kWave) out.sp_xyz[3 * a0 + i] = 0.0f;
+        for (int i = lane; i < n; i += kWave) {
+            out.sp_t[a0 + i] = 0.0f;
+            out.mask[a0 + i] = 0;
+        }
+        for (int i = lane; i < n * o.Kc; i += kWave) out.pc_idx[a0 * o.Kc + i] = -1;
+    } else {
         if (lane == 0) out.n_samples[ri] = ns;
         const int per = out.s_max - ns;  // empty slots, o.Kc entries each
         for (int i = lane; i < per * o.Kc; i += kWave) {
@@ -1161,9 +1170,12 @@ __device__ __forceinline__ void write_tail_wave(const SampleOpts& o, const Sampl
     }
 }
 
-template <int MODE, int KMAX, int SMAX, int LAY, int NW, class Src>
-__device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray, int u, int v,
-                             int W, int H, int64_t ri) {
+// SHARED_TAIL: every layout's tail is written by the whole wave (kernels that
+// also hold the one-lane form keep lane 0 writing it outside the feature
+// layout: sharing it there costs the one-lane form registers).
+template <int MODE, int KMAX, int SMAX, int LAY, int NW, bool SHARED_TAIL = false, class Src>
+__device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOut& out, const Src& src, const Ray& ray,
+                                             int u, int v, int W, int H, int64_t ri) {
     constexpr int NS = KMAX <= 8 ? 2 : 1;
     const int lane = __lane_id();
     // window lanes 64 c + lane (c < NW): their buckets and cone tests (same
@@ -1393,7 +1405,7 @@ __device__ __forceinline__ void run_ray_wave(const SampleOpts& o, const SampleOu
             }
         }
     }
-    if constexpr (LAY == kLayFeat) write_tail_wave<MODE>(o, out, ri, ns);
+    if constexpr (LAY == kLayFeat || SHARED_TAIL) write_tail_wave<MODE, LAY>(o, out, ri, ns);
     else if (lane == 0) write_tail<MODE, LAY>(o, out, ri, ns);
 }
 
