@@ -773,19 +773,24 @@ __device__ __forceinline__ void write_sample(const SampleOpts& o, const SampleOu
 
 // Every output element of ray ri not written for its ns emitted samples:
 // zeros / -1 / false, and the sample count.
+// Stores value into the n 4-byte words at p: single words up to a 16-byte
+// boundary, then 16 bytes at a time, then single words.
+__device__ __forceinline__ void fill_words(uint32_t* p, int n, uint32_t value) {
+    while (n > 0 && (reinterpret_cast<uintptr_t>(p) & 15)) *p++ = value, --n;
+    for (; n >= 4; n -= 4, p += 4) *reinterpret_cast<uint4*>(p) = make_uint4(value, value, value, value);
+    while (n > 0) *p++ = value, --n;
+}
+
 template <int MODE, int LAY>
 __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut& out, int64_t ri, int ns) {
     if constexpr (LAY == kLayDense) {
-        for (int s = ns; s < o.S; ++s) {
-            const int64_t at = ri * o.S + s;
-            out.sp_xyz[3 * at + 0] = 0.0f;
-            out.sp_xyz[3 * at + 1] = 0.0f;
-            out.sp_xyz[3 * at + 2] = 0.0f;
-            out.sp_t[at] = 0.0f;
-            out.mask[at] = 0;
-            int* dst = out.pc_idx + at * o.Kc;
-            for (int b = 0; b < o.Kc; ++b) dst[b] = -1;
-        }
+        // slots [ns, S) of the ray are contiguous in each output
+        const int64_t a0 = ri * o.S + ns;
+        const int n = o.S - ns;
+        fill_words(reinterpret_cast<uint32_t*>(out.sp_xyz + 3 * a0), 3 * n, 0u);
+        fill_words(reinterpret_cast<uint32_t*>(out.sp_t + a0), n, 0u);
+        fill_words(reinterpret_cast<uint32_t*>(out.pc_idx + a0 * o.Kc), n * o.Kc, 0xffffffffu);
+        for (int s = 0; s < n; ++s) out.mask[a0 + s] = 0;
     } else {
         out.n_samples[ri] = ns;
         for (int s = ns; s < out.s_max; ++s) {
