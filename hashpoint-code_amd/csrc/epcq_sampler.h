@@ -545,6 +545,14 @@ struct NbBuf {
     }
 };
 
+// Stores value into the n 4-byte words at p: single words up to a 16-byte
+// boundary, then 16 bytes at a time, then single words.
+__device__ __forceinline__ void fill_words(uint32_t* p, int n, uint32_t value) {
+    while (n > 0 && (reinterpret_cast<uintptr_t>(p) & 15)) *p++ = value, --n;
+    for (; n >= 4; n -= 4, p += 4) *reinterpret_cast<uint4*>(p) = make_uint4(value, value, value, value);
+    while (n > 0) *p++ = value, --n;
+}
+
 // The reference's 8-entry buffer (quick_search_nearby_pc, :50-73) for the
 // one-lane sampler when only the neighbour indices are output: the indices
 // go straight to the sample's output row as the buffer takes them (entry q
@@ -603,7 +611,7 @@ struct CompatStream {
 
     // -1 in the row's columns the buffer left empty.
     __device__ __forceinline__ void finish() const {
-        for (int q = n; q < kc; ++q) row[q] = -1;
+        if (n < kc) fill_words(reinterpret_cast<uint32_t*>(row + n), kc - n, 0xffffffffu);
     }
 };
 
@@ -773,14 +781,6 @@ __device__ __forceinline__ void write_sample(const SampleOpts& o, const SampleOu
 
 // Every output element of ray ri not written for its ns emitted samples:
 // zeros / -1 / false, and the sample count.
-// Stores value into the n 4-byte words at p: single words up to a 16-byte
-// boundary, then 16 bytes at a time, then single words.
-__device__ __forceinline__ void fill_words(uint32_t* p, int n, uint32_t value) {
-    while (n > 0 && (reinterpret_cast<uintptr_t>(p) & 15)) *p++ = value, --n;
-    for (; n >= 4; n -= 4, p += 4) *reinterpret_cast<uint4*>(p) = make_uint4(value, value, value, value);
-    while (n > 0) *p++ = value, --n;
-}
-
 template <int MODE, int LAY>
 __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut& out, int64_t ri, int ns) {
     if constexpr (LAY == kLayDense) {
@@ -790,7 +790,13 @@ __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut&
         fill_words(reinterpret_cast<uint32_t*>(out.sp_xyz + 3 * a0), 3 * n, 0u);
         fill_words(reinterpret_cast<uint32_t*>(out.sp_t + a0), n, 0u);
         fill_words(reinterpret_cast<uint32_t*>(out.pc_idx + a0 * o.Kc), n * o.Kc, 0xffffffffu);
-        for (int s = 0; s < n; ++s) out.mask[a0 + s] = 0;
+        // the mask bytes: single bytes up to a 4-byte boundary, then words
+        uint8_t* m = out.mask + a0;
+        int nb = n;
+        while (nb > 0 && (reinterpret_cast<uintptr_t>(m) & 3)) *m++ = 0, --nb;
+        fill_words(reinterpret_cast<uint32_t*>(m), nb >> 2, 0u);
+        m += nb & ~3;
+        for (int s = 0; s < (nb & 3); ++s) m[s] = 0;
     } else {
         out.n_samples[ri] = ns;
         for (int s = ns; s < out.s_max; ++s) {
