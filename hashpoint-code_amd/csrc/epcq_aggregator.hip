@@ -88,9 +88,15 @@ __device__ __forceinline__ void pe_double(float s, float c, float& s2, float& c2
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// v for v > 0, else 0.01 v (both branches agree on +-0, infinities and NaN):
-// one multiply and one max instead of a compare and a select as well.
-__device__ __forceinline__ float lrelu(float v) { return fmaxf(v, 0.01f * v); }
+// v for v > 0, else 0.01 v, as the larger of the two: one multiply and one
+// v_max_f32, written out because fmaxf of a matrix-core result adds a
+// NaN-quieting max of its own.  Equal to the compare-and-select form for every
+// finite v, +-0 and +-inf (a NaN stays a NaN).
+__device__ __forceinline__ float lrelu(float v) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(v), "v"(0.01f * v));
+    return r;
+}
 
 // torch.nn.Softplus(beta=1, threshold=20)
 __device__ __forceinline__ float softplus(float v) { return v > 20.0f ? v : log1pf(expf(v)); }
