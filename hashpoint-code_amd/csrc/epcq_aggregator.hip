@@ -397,11 +397,13 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
         float acc_a = 0.0f;
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int j = 0; j < Kp; ++j) {
-            // every lane reads the same source row: the branch is uniform
+            // every lane reads the same source row (a wave-uniform lane), so
+            // the values come from registers by readlane, with no LDS round
+            // trip, and the branch is uniform
             const int src = s * Kp + j;
-            const float wj = __shfl(w, src);
-            const float aj = __shfl(alpha_k, src);
-            const int vj = __shfl((int)valid, src);
+            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), src));
+            const float aj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(alpha_k), src));
+            const int vj = __builtin_amdgcn_readlane((int)valid, src);
             if (!vj) continue;
             acc_a += wj * aj;
             uint2 h;
