@@ -33,6 +33,8 @@
 // scheduling.
 #include <math.h>
 
+#include <type_traits>
+
 #include "epcq_common.h"
 
 namespace epcq {
@@ -66,8 +68,10 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
 // sine and cosine.  The rounding of x / 2 pi bounds the phase error by
 // |x| 2^-24 (6e-5 rad at |x| = 1024), far below the bf16 step of 2^-8 near
 // 1; the library's sincosf beyond.
+// CHECKED false: the caller has established |x| < 1024 (no branch here).
+template <bool CHECKED = true>
 __device__ __forceinline__ void pe_sincos(float x, float& s, float& c) {
-    if (fabsf(x) < 1024.0f) {
+    if (!CHECKED || fabsf(x) < 1024.0f) {
         float r = x * 0.15915494309189535f;
         r = r - rintf(r);
         s = __builtin_amdgcn_sinf(r);
@@ -287,38 +291,53 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
         __builtin_memcpy(xr + half * 16, &w0, 16);
         __builtin_memcpy(xr + half * 16 + 8, &w1, 16);
     }
+    // dist = p - s and distpow = p^2 - s^2; invalid pairs have p = 0 as the
+    // reference's zero-filled gathers
+    const float df[6] = {px - sx, py - sy, pz - sz, px * px - sx * sx, py * py - sy * sy, pz * pz - sz * sz};
+    auto encode = [&](auto checked) {
+        constexpr bool C = decltype(checked)::value;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        // PE(f, 3) of feature d = half * 16 + i: 6 values at 32 + 6 d
-        float sn[3], cs[3];
-        pe_sincos(fs[i], sn[0], cs[0]);
-        pe_double(sn[0], cs[0], sn[1], cs[1]);
-        pe_double(sn[1], cs[1], sn[2], cs[2]);
-        uint32_t* o = reinterpret_cast<uint32_t*>(xr + kFeat + 6 * (half * 16 + i));
-        o[0] = pack_bf16(sn[0], cs[0]);
-        o[1] = pack_bf16(sn[1], cs[1]);
-        o[2] = pack_bf16(sn[2], cs[2]);
-    }
-    if (half) {
-        // dist = p - s and distpow = p^2 - s^2; invalid pairs have p = 0 as
-        // the reference's zero-filled gathers
-        const float df[6] = {px - sx, py - sy, pz - sz, px * px - sx * sx, py * py - sy * sy, pz * pz - sz * sz};
-        uint32_t* o = reinterpret_cast<uint32_t*>(xr + 224);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            // frequencies 1 and 4 directly; 2, 8 and 16 by doubling
-            float sn[5], cs[5];
-            pe_sincos(df[i], sn[0], cs[0]);
-            pe_sincos(df[i] * 4.0f, sn[2], cs[2]);
+        for (int i = 0; i < 16; ++i) {
+            // PE(f, 3) of feature d = half * 16 + i: 6 values at 32 + 6 d
+            float sn[3], cs[3];
+            pe_sincos<C>(fs[i], sn[0], cs[0]);
             pe_double(sn[0], cs[0], sn[1], cs[1]);
-            pe_double(sn[2], cs[2], sn[3], cs[3]);
-            pe_double(sn[3], cs[3], sn[4], cs[4]);
-#pragma unroll
-            for (int j = 0; j < 5; ++j) o[i * 5 + j] = pack_bf16(sn[j], cs[j]);
+            pe_double(sn[1], cs[1], sn[2], cs[2]);
+            uint32_t* o = reinterpret_cast<uint32_t*>(xr + kFeat + 6 * (half * 16 + i));
+            o[0] = pack_bf16(sn[0], cs[0]);
+            o[1] = pack_bf16(sn[1], cs[1]);
+            o[2] = pack_bf16(sn[2], cs[2]);
         }
-        o[30] = 0;  // columns 284..287
-        o[31] = 0;
+        if (half) {
+            uint32_t* o = reinterpret_cast<uint32_t*>(xr + 224);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                // frequencies 1 and 4 directly; 2, 8 and 16 by doubling
+                float sn[5], cs[5];
+                pe_sincos<C>(df[i], sn[0], cs[0]);
+                pe_sincos<C>(df[i] * 4.0f, sn[2], cs[2]);
+                pe_double(sn[0], cs[0], sn[1], cs[1]);
+                pe_double(sn[2], cs[2], sn[3], cs[3]);
+                pe_double(sn[3], cs[3], sn[4], cs[4]);
+#pragma unroll
+                for (int j = 0; j < 5; ++j) o[i * 5 + j] = pack_bf16(sn[j], cs[j]);
+            }
+            o[30] = 0;  // columns 284..287
+            o[31] = 0;
+        }
+    };
+    // one test for the wave: when every argument of its encodings is below
+    // 1024 in magnitude (as it is for any features of ordinary scale) they
+    // all take the hardware path without a branch each
+    bool large = false;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) large = large || !(fabsf(fs[i]) < 1024.0f);
+    if (half) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) large = large || !(fabsf(df[i] * 4.0f) < 1024.0f);
     }
+    if (__ballot(large) == 0) encode(std::false_type{});
+    else encode(std::true_type{});
     const bool any_valid = __ballot(valid) != 0;
     block_layer<kX1 / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w1), a.p.b1, any_valid);
     block_layer<kH / 16>(io, s_w, static_cast<const uint16_t*>(a.p.w2), a.p.b2, any_valid);
