@@ -196,9 +196,9 @@ __device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const u
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = (f32x16)(bias[t * 32 + row]);
     const uint16_t* a_row = io + row * kLdsStride + kh;
-    // unrolled by two so that the two chunks in flight swap registers
-    // instead of being copied each step
-#pragma unroll 2
+    // unrolled so that the two chunks in flight swap registers instead of
+    // being copied each step (four steps measured faster than two)
+#pragma unroll 4
     for (int ks = 0; ks < KS; ++ks) {
         __syncthreads();  // the previous chunk has been read
         __builtin_memcpy(s_w + tid * 8, &c0, 16);
