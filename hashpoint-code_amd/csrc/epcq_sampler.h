@@ -421,6 +421,7 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
         if (e.y <= 0) return;
         int kept = 0;
         float4 nxt = src.point(e.x);
+#pragma unroll 2
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
             const float4 p = nxt;
             if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
