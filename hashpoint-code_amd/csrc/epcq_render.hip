@@ -129,7 +129,13 @@ __device__ __forceinline__ void render_row(const float* tr, const float* dr, con
 // dL/dsigma to gs[k] and dL/dc to gc[3k..3k+2].  gs and gc may be the rows sr
 // and cr themselves: sample k's gradients are written after its inputs are
 // read, and no later sample reads them.
-template <int CH>
+// With CACHE (rows in LDS, writable): pass 1 leaves each sample's interval in
+// the t row (t given) and exp(-sigma delta) in the sigma row, and pass 2
+// reads them back instead of recomputing them (the same values: a stored
+// float reloads with its bits).  gs and gc may be the rows sr and cr
+// themselves: sample k's gradients are written after its inputs are read,
+// and no later sample reads them.
+template <int CH, bool CACHE = false>
 __device__ __forceinline__ void render_row_backward(const float* tr, const float* dr, const float* sr, const float* cr,
                                                     int S, float vsize, float bg, float g0, float g1, float g2,
                                                     float* gs, float* gc) {
@@ -147,11 +153,19 @@ __device__ __forceinline__ void render_row_backward(const float* tr, const float
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             if (j < n) {
-                const float op = 1.0f - expf_cw(-ch.s[j] * ch.d[j]);
+                const float e = expf_cw(-ch.s[j] * ch.d[j]);
+                const float op = 1.0f - e;
                 const float gc = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
                 total = total + op * T * gc;
                 Tlast = T;
                 T = T * ((1.0f - op) + 1e-10f);
+                if constexpr (CACHE) {
+                    // the chunk's t were read before any is replaced, and the
+                    // next chunk's first t (its own and this chunk's tnext)
+                    // is not replaced until that chunk is processed
+                    if (tr) const_cast<float*>(tr)[k0 + j] = ch.d[j];
+                    const_cast<float*>(sr)[k0 + j] = e;
+                }
             }
         }
     }
@@ -159,16 +173,30 @@ __device__ __forceinline__ void render_row_backward(const float* tr, const float
     // pass 2: per-sample gradients with the suffix sums
     T = 1.0f;
     float prefix = 0.0f;
+    const float* drow = CACHE && tr ? tr : dr;  // the intervals, when cached or given
     for (int k0 = 0; k0 < S; k0 += CH) {
         const int n = min(CH, S - k0);
         RowChunk<CH> ch;
-        load_chunk(tr, dr, sr, cr, S, k0, n, vsize, ch);
+        if constexpr (CACHE) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                if (j < n) {
+                    ch.d[j] = drow[k0 + j];
+                    ch.s[j] = sr[k0 + j];  // exp(-sigma delta)
+                    ch.c[3 * j] = cr[3 * (k0 + j)];
+                    ch.c[3 * j + 1] = cr[3 * (k0 + j) + 1];
+                    ch.c[3 * j + 2] = cr[3 * (k0 + j) + 2];
+                }
+            }
+        } else {
+            load_chunk(tr, dr, sr, cr, S, k0, n, vsize, ch);
+        }
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
             if (j < n) {
                 const int k = k0 + j;
                 const float dk = ch.d[j];
-                const float e = expf_cw(-ch.s[j] * dk);
+                const float e = CACHE ? ch.s[j] : expf_cw(-ch.s[j] * dk);
                 const float op = 1.0f - e;
                 const float a = (1.0f - op) + 1e-10f;
                 const float gck = dot3(g0, g1, g2, ch.c[3 * j], ch.c[3 * j + 1], ch.c[3 * j + 2]);
@@ -337,7 +365,7 @@ __global__ __launch_bounds__(kStageRays) void render_backward_staged_kernel(
     if (l < nr) {
         const int64_t r = r0 + l;
         // the gradients replace the row's sigma and rgb in LDS
-        render_row_backward<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
+        render_row_backward<kLdsChunk, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
                                        sc + l * wc, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
                                        grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
     }
@@ -541,7 +569,7 @@ __global__ __launch_bounds__(kStageRays) void render_backward_vec_kernel(
     const int l = threadIdx.x;
     if (l < nr) {
         const int64_t r = r0 + l;
-        render_row_backward<kLdsChunk>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
+        render_row_backward<kLdsChunk, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
                                        sc + l * wc, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
                                        grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
     }
