@@ -389,10 +389,12 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
             const float4 wq = wa4[q];
             uint2 h;
             __builtin_memcpy(&h, hr + 4 * q, sizeof(h));
-            part += wq.x * __uint_as_float(h.x << 16);
-            part += wq.y * __uint_as_float(h.x & 0xffff0000u);
-            part += wq.z * __uint_as_float(h.y << 16);
-            part += wq.w * __uint_as_float(h.y & 0xffff0000u);
+            // fused multiply-adds (one rounding each; the module's fp32 sums are
+            // matched within the bf16 tolerance either way)
+            part = __builtin_fmaf(wq.x, __uint_as_float(h.x << 16), part);
+            part = __builtin_fmaf(wq.y, __uint_as_float(h.x & 0xffff0000u), part);
+            part = __builtin_fmaf(wq.z, __uint_as_float(h.y << 16), part);
+            part = __builtin_fmaf(wq.w, __uint_as_float(h.y & 0xffff0000u), part);
         }
     }
     const float other = __shfl_xor(part, 32);
@@ -426,13 +428,13 @@ __global__ __launch_bounds__(kPairBlock) void agg_pair_kernel(PairArgs a) {
             const float aj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(alpha_k), src));
             const int vj = __builtin_amdgcn_readlane((int)valid, src);
             if (!vj) continue;
-            acc_a += wj * aj;
+            acc_a = __builtin_fmaf(wj, aj, acc_a);
             uint2 h;
             __builtin_memcpy(&h, io + src * kLdsStride + lane * 4, sizeof(h));
-            acc[0] += wj * __uint_as_float(h.x << 16);
-            acc[1] += wj * __uint_as_float(h.x & 0xffff0000u);
-            acc[2] += wj * __uint_as_float(h.y << 16);
-            acc[3] += wj * __uint_as_float(h.y & 0xffff0000u);
+            acc[0] = __builtin_fmaf(wj, __uint_as_float(h.x << 16), acc[0]);
+            acc[1] = __builtin_fmaf(wj, __uint_as_float(h.x & 0xffff0000u), acc[1]);
+            acc[2] = __builtin_fmaf(wj, __uint_as_float(h.y << 16), acc[2]);
+            acc[3] = __builtin_fmaf(wj, __uint_as_float(h.y & 0xffff0000u), acc[3]);
         }
         if (ms >= a.M) continue;
         uint16_t* xo = a.xc + ms * kXc;
