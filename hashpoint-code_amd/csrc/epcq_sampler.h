@@ -800,19 +800,19 @@ __device__ __forceinline__ void write_tail(const SampleOpts& o, const SampleOut&
         for (int s = 0; s < (nb & 3); ++s) m[s] = 0;
     } else {
         out.n_samples[ri] = ns;
-        for (int s = ns; s < out.s_max; ++s) {
-            const int64_t at = ri * out.s_max + s;
-            out.sample_t[at] = 0.0f;
-            for (int b = 0; b < o.Kc; ++b) {
-                out.nb_idx[at * o.Kc + b] = -1;
-                if constexpr (MODE == EPCQ_MODE_KNN) {
-                    if (out.nb_t) out.nb_t[at * o.Kc + b] = 0.0f;
-                    if (out.nb_dist) out.nb_dist[at * o.Kc + b] = 0.0f;
-                }
-                if constexpr (LAY == kLayFeat) out.w[at * o.Kc + b] = 0.0f;
-            }
-            if constexpr (LAY == kLayFeat)
-                for (int f = 0; f < o.F; ++f) out.agg[at * o.F + f] = 0.0f;
+        if (ns >= out.s_max) return;
+        // slots [ns, s_max) of the ray are contiguous in each output
+        const int64_t a0 = ri * out.s_max + ns;
+        const int n = out.s_max - ns;
+        fill_words(reinterpret_cast<uint32_t*>(out.sample_t + a0), n, 0u);
+        fill_words(reinterpret_cast<uint32_t*>(out.nb_idx + a0 * o.Kc), n * o.Kc, 0xffffffffu);
+        if constexpr (MODE == EPCQ_MODE_KNN) {
+            if (out.nb_t) fill_words(reinterpret_cast<uint32_t*>(out.nb_t + a0 * o.Kc), n * o.Kc, 0u);
+            if (out.nb_dist) fill_words(reinterpret_cast<uint32_t*>(out.nb_dist + a0 * o.Kc), n * o.Kc, 0u);
+        }
+        if constexpr (LAY == kLayFeat) {
+            fill_words(reinterpret_cast<uint32_t*>(out.w + a0 * o.Kc), n * o.Kc, 0u);
+            fill_words(reinterpret_cast<uint32_t*>(out.agg + a0 * o.F), n * o.F, 0u);
         }
     }
 }
