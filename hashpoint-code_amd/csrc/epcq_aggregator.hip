@@ -215,17 +215,22 @@ __device__ __forceinline__ void block_layer(uint16_t* io, uint16_t* s_w, const u
         }
     }
     if (!active) return;
+    // Rows rr and rr + 1 of column c are converted together; each even lane
+    // then swaps halves with the odd lane beside it (column c + 1), so that
+    // the even lane stores row rr and the odd lane row rr + 1 of the column
+    // pair (c & ~1, c | 1) as one 32-bit word.
     const int r0 = (lane >> 5) * 4;
+    const bool odd = lane & 1;
+    const uint32_t sel = odd ? 0x03020706u : 0x05040100u;  // bytes of (neighbour, own)
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int c = t * 32 + row;
+        const int c = t * 32 + (row & ~1);
 #pragma unroll
         for (int r = 0; r < 16; r += 2) {
-            // rows rr and rr + 1 of column c: one conversion for both
-            const int rr = (r >> 2) * 8 + r0 + (r & 3);
+            const int rr = (r >> 2) * 8 + r0 + (r & 3) + (odd ? 1 : 0);
             const uint32_t p = pack_bf16(lrelu(acc[t][r]), lrelu(acc[t][r + 1]));
-            io[rr * kLdsStride + c] = (uint16_t)p;
-            io[(rr + 1) * kLdsStride + c] = (uint16_t)(p >> 16);
+            const uint32_t q = (uint32_t)__builtin_amdgcn_mov_dpp((int)p, 0xB1, 0xF, 0xF, false);
+            *reinterpret_cast<uint32_t*>(io + rr * kLdsStride + c) = __builtin_amdgcn_perm(q, p, sel);
         }
     }
 }
