@@ -303,7 +303,7 @@ struct ViewLayout {
 
 ViewLayout layout_view(const epcq_view_desc& d) {
     ViewLayout L{};
-    L.bp = plan_build(d.n, d.height, d.width, true);
+    L.bp = plan_build(d.n, d.height, d.width, true, kBatchBinPoints);
     L.P = (int64_t)d.width * d.height;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -337,7 +337,7 @@ int check_views(const epcq_view_desc* views, int n_views) {
         if (d.width <= 0 || d.height <= 0) return fail(EPCQ_ERR_INVALID, "batch: image size must be positive");
         if ((int64_t)d.width * d.height >= (int64_t(1) << 30)) return fail(EPCQ_ERR_INVALID, "batch: image too large");
         if (d.n < 0 || d.n >= (int64_t(1) << 31)) return fail(EPCQ_ERR_INVALID, "batch: point count out of range");
-        if ((1 << plan_build(d.n, d.height, d.width).ts) > kMaxFineTile)
+        if ((1 << plan_build(d.n, d.height, d.width, true, kBatchBinPoints).ts) > kMaxFineTile)
             return fail(EPCQ_ERR_INVALID, "batch: image too large for the batched build");
     }
     return EPCQ_OK;

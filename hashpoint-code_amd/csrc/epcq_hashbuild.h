@@ -292,14 +292,16 @@ struct BuildPlan {
 
 // points: the bin pass carries each point's coordinates with its index (a
 // 16-byte record) so that the fine pass can write the pixel-ordered point
-// copy without gathering from xyz; no depth is binned then.
-inline BuildPlan plan_build(int64_t n, int H, int W, bool points = false) {
+// copy without gathering from xyz; no depth is binned then.  bin_points: the
+// points a bin of a dense view is sized to hold (the fine pass gives each bin
+// one block).
+inline BuildPlan plan_build(int64_t n, int H, int W, bool points = false, int bin_points = kBinPoints) {
     BuildPlan bp{};
     bp.P = (int64_t)H * W;
     // bins of at most 2^kMaxTileShift pixels, fewer for dense views so that a
     // bin holds about kBinPoints points or less (the fine pass gives each bin
     // one block), but never more than 2^kMaxBinBits bins
-    const int64_t q = (int64_t)kBinPoints * bp.P / std::max<int64_t>(n, 1);
+    const int64_t q = (int64_t)bin_points * bp.P / std::max<int64_t>(n, 1);
     int ts_dense = 0;
     while (ts_dense < kMaxTileShift && (int64_t(2) << ts_dense) <= q) ++ts_dense;
     bp.ts = std::max(ceil_log2(bp.P) - kMaxBinBits, ts_dense);
