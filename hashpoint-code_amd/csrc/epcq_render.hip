@@ -52,11 +52,12 @@ __device__ __forceinline__ float clamp_delta_v(float tk, float tk1, bool last, f
 // Samples [k0, k0 + n) of a row (n <= CH) in registers: their
 // intervals, densities and colours.  All loads of the chunk are issued before
 // any is used, so a lane waits for memory once per chunk, not once per sample.
-// CH = kChunk for rows in global memory; the LDS-staged forms use
-// kLdsChunk (LDS answers quickly, and fewer registers leave room for the
-// next group's loads).
+// CH = kChunk for rows in global memory; the LDS-staged forms use the
+// smaller kLdsChunk and kLdsChunkBwd (LDS answers quickly; the sizes are the
+// measured best of 4 and 8 for each walk).
 constexpr int kChunk = 16;
-constexpr int kLdsChunk = 4;
+constexpr int kLdsChunk = 8;      // forward walk
+constexpr int kLdsChunkBwd = 4;   // backward walks (more live values per sample)
 template <int CH>
 struct RowChunk {
     float d[CH], s[CH], c[3 * CH];
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(kStageRays) void render_backward_staged_kernel(
     if (l < nr) {
         const int64_t r = r0 + l;
         // the gradients replace the row's sigma and rgb in LDS
-        render_row_backward<kLdsChunk, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
+        render_row_backward<kLdsChunkBwd, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
                                        sc + l * wc, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
                                        grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
     }
@@ -569,7 +570,7 @@ __global__ __launch_bounds__(kStageRays) void render_backward_vec_kernel(
     const int l = threadIdx.x;
     if (l < nr) {
         const int64_t r = r0 + l;
-        render_row_backward<kLdsChunk, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
+        render_row_backward<kLdsChunkBwd, true>(t ? st + l * ws : nullptr, t ? nullptr : st + l * ws, ss + l * ws,
                                        sc + l * wc, S, vsize, bg, grad_out[3 * r], grad_out[3 * r + 1],
                                        grad_out[3 * r + 2], ss + l * ws, sc + l * wc);
     }
