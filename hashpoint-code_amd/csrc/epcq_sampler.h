@@ -121,6 +121,7 @@ __device__ __forceinline__ float point_threshold(float r) {
 template <bool PR>
 struct SrcGlobal {
     static constexpr bool kPR = PR;
+    static constexpr bool kReadPast = false;  // see SrcLds
     const int2* hash;
     const int* sorted_idx;
     const float* xyz;
@@ -147,6 +148,12 @@ struct SrcGlobal {
 template <bool PR>
 struct SrcLds {
     static constexpr bool kPR = PR;
+    // Point j = e.x + e.y, one past a bucket, may be read (and not used): a
+    // tile is staged only when its halo holds at most `cap` points, the
+    // buckets' rebased ranges lie within [0, total), and the cap-entry point
+    // array is followed by the cone array in the same allocation, so index
+    // total <= cap stays inside it.  The walks then read ahead without a test.
+    static constexpr bool kReadPast = true;
     const int2* hash;
     const float4* pts;
     const float* prad;  // staged radius thresholds (per-point radius only)
@@ -167,6 +174,7 @@ struct SrcLds {
 template <bool PR>
 struct SrcHybrid {
     static constexpr bool kPR = PR;
+    static constexpr bool kReadPast = false;  // see SrcLds
     const int2* hash;  // LDS, halo region
     const float4* cone_a;
     const float2* cone_b;
@@ -424,7 +432,7 @@ __device__ __forceinline__ void for_each_ray_hit(const SampleOpts& o, const Src&
 #pragma unroll 2
         for (int j = 0; j < e.y && kept < kSampleLaneHits; ++j) {
             const float4 p = nxt;
-            if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
+            if (Src::kReadPast || j + 1 < e.y) nxt = src.point(e.x + j + 1);
             const float lim = point_limit(o, src, e.x + j, p);
             float t, e2;
             ray_t_e2(ray, p, t, e2);
@@ -643,7 +651,7 @@ __device__ __forceinline__ void neighbour_pass(const SampleOpts& o, const Src& s
         float4 nxt = src.point(e.x);
         for (int j = 0; j < e.y; ++j) {
             const float4 p = nxt;
-            if (j + 1 < e.y) nxt = src.point(e.x + j + 1);
+            if (Src::kReadPast || j + 1 < e.y) nxt = src.point(e.x + j + 1);
             visit(p, e.x + j);
         }
     };
