@@ -11,10 +11,12 @@
 //   1. count    each 256-thread block projects a chunk of points and builds an
 //               LDS histogram over coarse bins (runs of 2^ts consecutive
 //               row-major pixels);
-//   2. scan     one block per bin scans its column of the block histograms
-//               (each (block, bin) gets its cursor within the bin, each bin
-//               its total), then one block scans the totals into the bins'
-//               starts in the output, which the bin pass adds;
+//   2. scan     small count matrices: one block per bin scans its column of
+//               the block histograms (each (block, bin) gets its cursor
+//               within the bin, each bin its total), then one block scans the
+//               totals into the bins' starts in the output, which the bin
+//               pass adds.  Large ones: segment sums, bin totals, bin starts,
+//               then absolute cursors (four small kernels over rows of bins);
 //   3. bin      blocks re-project their chunk and append each point to its
 //               bin; lanes with equal bins are matched with ballots and waves
 //               take their turn in order, so each bin receives its points in
@@ -43,6 +45,18 @@ __global__ __launch_bounds__(kBlock) void build_count_kernel(BuildView bv) {
 __global__ __launch_bounds__(kBlock) void col_scan_kernel(BuildView bv) {
     __shared__ int s_wave[kBuildWaves];
     col_scan_body(bv, blockIdx.x, s_wave);
+}
+
+__global__ __launch_bounds__(kBlock) void col_partial_kernel(BuildView bv) {
+    col_partial_body(bv, blockIdx.x * kBlock + threadIdx.x, blockIdx.y);
+}
+
+__global__ __launch_bounds__(kBlock) void bin_total_kernel(BuildView bv) {
+    bin_total_body(bv, blockIdx.x * kBlock + threadIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void col_apply_kernel(BuildView bv) {
+    col_apply_body(bv, blockIdx.x * kBlock + threadIdx.x, blockIdx.y);
 }
 
 __global__ __launch_bounds__(kBlock) void bin_scan_kernel(BuildView bv) {
@@ -221,14 +235,32 @@ int build_hashtable_impl(const float* xyz, int64_t n, const float* w2c, const fl
     if (n == 0) return fill_launch(out_hash, 2 * bp.P, -1, st);
     BuildView bv = make_build_view(bp, xyz, n, w2c, K, height, width, sort, workspace, out_idx, out_depth,
                                    out_hash);
-    bv.rel_cursors = 1;  // one block per bin scans its column; binning adds the bin's start
+    // One block per bin scanning its column is one launch in place of three and
+    // wins on small block x bin count matrices (1M points at 800x800, 245 x
+    // 1250: 6 against 31 us); its strided column reads lose on large ones (20M
+    // points at 1920x1080, 1628 x 8100: 424 against 47 us), which take the
+    // segmented scan.
+    const bool one_pass = (int64_t)bp.nblk * bp.nbins <= kColScanEntries;
+    bv.rel_cursors = one_pass ? 1 : 0;  // the one-pass scan leaves cursors relative to the bin's start
     const size_t lds_bins = (size_t)bp.nbins * 4;
     build_count_kernel<<<dim3(bp.nblk), dim3(kBlock), lds_bins, st>>>(bv);
     if (int e = check_launch("build_hashtable/count")) return e;
-    col_scan_kernel<<<dim3(bp.nbins), dim3(kBlock), 0, st>>>(bv);
-    if (int e = check_launch("build_hashtable/col_scan")) return e;
-    bin_scan_kernel<<<dim3(1), dim3(kBlock), 0, st>>>(bv);
-    if (int e = check_launch("build_hashtable/bin_scan")) return e;
+    if (one_pass) {
+        col_scan_kernel<<<dim3(bp.nbins), dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/col_scan")) return e;
+        bin_scan_kernel<<<dim3(1), dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/bin_scan")) return e;
+    } else {
+        const dim3 cgrid((unsigned)ceil_div(bp.nbins, kBlock), (unsigned)bp.nseg);
+        col_partial_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/col_partial")) return e;
+        bin_total_kernel<<<dim3(cgrid.x), dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/bin_total")) return e;
+        bin_scan_kernel<<<dim3(1), dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/bin_scan")) return e;
+        col_apply_kernel<<<cgrid, dim3(kBlock), 0, st>>>(bv);
+        if (int e = check_launch("build_hashtable/col_apply")) return e;
+    }
     build_bin_kernel<<<dim3(bp.nblk), dim3(kBlock), lds_bins, st>>>(bv);
     if (int e = check_launch("build_hashtable/bin")) return e;
     const size_t lds_fine = (size_t)2 * (size_t(1) << bp.ts) * 4;

@@ -295,6 +295,10 @@ struct BuildPlan {
 // copy without gathering from xyz; no depth is binned then.  bin_points: the
 // points a bin of a dense view is sized to hold (the fine pass gives each bin
 // one block).
+// Largest block x bin count matrix the per-view build scans with one block
+// per bin (larger ones take the segmented scan).
+constexpr int64_t kColScanEntries = int64_t(1) << 20;
+
 inline BuildPlan plan_build(int64_t n, int H, int W, bool points = false, int bin_points = kBinPoints) {
     BuildPlan bp{};
     bp.P = (int64_t)H * W;
